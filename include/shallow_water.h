@@ -1,0 +1,103 @@
+/* shallow_water.h -- C ABI of the MI355X (gfx950) implementation of the nhswps per-timestep hot
+ * path.  Each entry point replaces one public function of the reference crate
+ * (rse-standrewscs/shallow-water); the citation next to it names the Rust function it stands in
+ * for.  All arrays use the reference's layout: element (i,j,iz) of an (ng,ng,nz+1) field lives at
+ * i + ng*j + ng*ng*iz (src/utils.rs:26-31); physical fields have i = y, j = x, spectral fields
+ * i = kx, j = ky with the stafft Hermitian packing.
+ *
+ * Pointers: every double* array argument may be a host pointer or a device pointer (HIP memory
+ * on the library's device).  Host arrays are staged through device memory; device arrays are used
+ * in place.  Unlike the reference, inputs documented as "destroyed" there are left intact here.
+ *
+ * Errors: functions return 0 or a negative sw_status instead of the reference's exit(1)/panic.
+ * Thread safety: per handle; calls on one handle must not overlap.
+ */
+#ifndef SHALLOW_WATER_H
+#define SHALLOW_WATER_H
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+    SW_OK = 0,
+    SW_ERR_ARG = -1,              /* shape mismatch / bad argument (reference: assert_eq! panic) */
+    SW_ERR_PSOLVE_MAXITER = -2,   /* psolve iteration cap reached (reference: psolve.rs:360-366 exit(1)) */
+    SW_ERR_PSOLVE_DIVERGED = -3,  /* psolve error > 1 after the first iteration (psolve.rs:349-352 exit(1)) */
+    SW_ERR_FACTOR = -4,           /* length not a product of 2,3,4,5,6 (stafft/mod.rs:148-149 exit(1)) */
+    SW_ERR_UNSUPPORTED = -5,      /* length the reference itself cannot transform correctly */
+    SW_ERR_HIP = -6,              /* HIP runtime error */
+    SW_ERR_NONFINITE = -7         /* NaN/Inf reached the psolve residual */
+} sw_status;
+
+typedef enum { SW_FP64 = 0, SW_MIXED = 1 } sw_precision;
+
+/* Field ids of a state, in the order of the reference State (src/nhswps/mod.rs:87-133). */
+typedef enum {
+    SW_U = 0, SW_V, SW_W, SW_Z, SW_ZX, SW_ZY, SW_R, SW_RI, SW_AA, SW_ZETA,
+    SW_PN, SW_DPN, SW_PS, SW_QS, SW_DS, SW_GS, SW_NFIELDS
+} sw_field;
+
+typedef struct sw_d2fft sw_d2fft;
+typedef struct sw_spectral sw_spectral;
+typedef struct sw_state sw_state;
+
+/* ---- library ---- */
+int sw_init(int device);            /* select the HIP device used by later calls (default 0) */
+int sw_sync(void);                  /* wait for all work queued by the library */
+const char* sw_last_error(void);    /* text of the last error on this thread */
+
+/* ---- 1-D FFT: src/stafft/mod.rs ---- */
+int sw_factorisen(size_t n, size_t factors[5]);                       /* factorisen  mod.rs:97  */
+int sw_initfft(size_t n, size_t factors[5], double* trig);            /* initfft     mod.rs:67  (host) */
+int sw_forfft(size_t m, size_t n, double* x, const double* trig,      /* forfft      mod.rs:158 */
+              const size_t factors[5]);
+int sw_revfft(size_t m, size_t n, double* x, const double* trig,      /* revfft      mod.rs:301 */
+              const size_t factors[5]);
+
+/* ---- 2-D FFT and spectral derivatives: src/sta2dfft.rs ---- */
+int sw_d2fft_create(size_t nx, size_t ny, double lx, double ly, double* kx, double* ky,
+                    sw_d2fft** out);                                  /* D2FFT::new   :37 */
+int sw_d2fft_destroy(sw_d2fft* d);
+int sw_ptospc(const sw_d2fft* d, double* rvar, double* svar);         /* ptospc  :91  */
+int sw_spctop(const sw_d2fft* d, double* svar, double* rvar);         /* spctop  :113 */
+int sw_xderiv(const sw_d2fft* d, const double* rkx, const double* var, double* der); /* :128 */
+int sw_yderiv(const sw_d2fft* d, const double* rky, const double* var, double* der); /* :157 */
+
+/* ---- spectral operators and inversion: src/spectral/mod.rs ---- */
+int sw_spectral_create(size_t ng, size_t nz, int precision, sw_spectral** out); /* Spectral::new :65 */
+int sw_spectral_destroy(sw_spectral* s);
+long sw_spectral_get(const sw_spectral* s, const char* name, double* out); /* operator export (host) */
+int sw_main_invert(sw_spectral* s, const double* qs, const double* ds, const double* gs,
+                   double* r, double* u, double* v, double* zeta);    /* main_invert :305 */
+int sw_jacob(sw_spectral* s, const double* aa, const double* bb, double* cs);   /* jacob :423 */
+int sw_divs(sw_spectral* s, const double* aa, const double* bb, double* cs);    /* divs  :461 */
+int sw_ptospc3d(sw_spectral* s, const double* fp, double* fs, size_t izbeg, size_t izend); /* :479 */
+int sw_spctop3d(sw_spectral* s, const double* fs, double* fp, size_t izbeg, size_t izend); /* :502 */
+int sw_deal3d(sw_spectral* s, double* fp);                            /* deal3d :524 */
+int sw_deal2d(sw_spectral* s, double* fp);                            /* deal2d :544 */
+
+/* ---- state and the nhswps kernels: src/nhswps ---- */
+int sw_state_create(sw_spectral* s, sw_state** out);                  /* State (mod.rs:87) */
+int sw_state_destroy(sw_state* st);
+int sw_state_set(sw_state* st, int field, const double* src);
+int sw_state_get(sw_state* st, int field, double* dst);
+int sw_state_set_time(sw_state* st, double t, int ggen);
+int sw_state_get_time(const sw_state* st, double* t);
+int sw_vertical(sw_state* st);                                        /* vertical.rs:8  */
+int sw_coeffs(sw_state* st, double* sigx, double* sigy, double* cpt1, double* cpt2); /* coeffs.rs:8 */
+int sw_cpsource(sw_state* st, double* sp0);                           /* cpsource.rs:13 */
+int sw_psolve(sw_state* st, int nitmax, int* iters, double* errp);    /* psolve.rs:16   */
+int sw_source(sw_state* st, double* sqs, double* sds, double* sgs);   /* source.rs:19   */
+int sw_diagnose(sw_state* st, double diag[5]);                        /* diagnose mod.rs:666 */
+int sw_advance(sw_state* st, double diag[5], int iters[3]);           /* advance.rs:13  */
+
+/* ---- synthetic initial state (host, no reference counterpart: the bench/test generator) ---- */
+int sw_synth_kminus3(size_t ng, size_t nz, unsigned long long seed, double k0, double zeta_rms,
+                     double delta_rms, double* qq, double* dd);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

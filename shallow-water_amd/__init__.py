@@ -1,0 +1,261 @@
+"""shallow-water_amd: the nhswps per-timestep hot path on MI355X (gfx950).
+
+A thin ctypes layer over libsw_hip.so (C ABI in include/shallow_water.h) that mirrors the reference
+crate's API for this path: stafft (factorisen, initfft, forfft, revfft), sta2dfft (D2FFT with
+ptospc, spctop, xderiv, yderiv), spectral (Spectral with main_invert, jacob, divs, ptospc3d,
+spctop3d, deal3d, deal2d) and nhswps (State with vertical, coeffs, cpsource, psolve, source,
+diagnose, advance).  Arrays are float64 numpy arrays in the reference layout (element (i,j,iz) at
+i + ng*j + ng*ng*iz).  Errors raise SwError carrying the library's sw_status code where the
+reference would exit(1) or panic.
+
+The library must be built (make -C shallow-water_amd, or __graft_entry__.build()); importing this
+module without it raises ImportError: there is no CPU fallback.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libsw_hip.so")
+
+FP64, MIXED = 0, 1
+FIELDS = ["u", "v", "w", "z", "zx", "zy", "r", "ri", "aa", "zeta", "pn", "dpn", "ps", "qs", "ds", "gs"]
+STATUS = {0: "SW_OK", -1: "SW_ERR_ARG", -2: "SW_ERR_PSOLVE_MAXITER", -3: "SW_ERR_PSOLVE_DIVERGED",
+          -4: "SW_ERR_FACTOR", -5: "SW_ERR_UNSUPPORTED", -6: "SW_ERR_HIP", -7: "SW_ERR_NONFINITE"}
+
+
+class SwError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"{STATUS.get(code, code)}: {msg}")
+        self.code = code
+
+
+if not os.path.exists(LIB_PATH):
+    raise ImportError(f"{LIB_PATH} is not built (run make -C shallow-water_amd)")
+_lib = C.CDLL(LIB_PATH)
+_lib.sw_last_error.restype = C.c_char_p
+_lib.sw_spectral_get.restype = C.c_long
+_dp = C.POINTER(C.c_double)
+_sp = C.POINTER(C.c_size_t)
+_vp = C.c_void_p
+
+
+def _check(rc):
+    if rc != 0:
+        raise SwError(rc, _lib.sw_last_error().decode())
+    return rc
+
+
+def _d(a):
+    if not (isinstance(a, np.ndarray) and a.dtype == np.float64 and a.flags.c_contiguous):
+        raise TypeError("expected a C-contiguous float64 numpy array")
+    return a.ctypes.data_as(_dp)
+
+
+def _sz(f):
+    a = np.ascontiguousarray(f, dtype=np.uint64)
+    return a, a.ctypes.data_as(_sp)
+
+
+def lib_path():
+    return LIB_PATH
+
+
+def init(device=0):
+    _check(_lib.sw_init(C.c_int(device)))
+
+
+def sync():
+    _check(_lib.sw_sync())
+
+
+# ---------------------------------------------------------------- stafft (src/stafft/mod.rs)
+def factorisen(n):
+    f = np.zeros(5, np.uint64)
+    _check(_lib.sw_factorisen(C.c_size_t(n), f.ctypes.data_as(_sp)))
+    return [int(x) for x in f]
+
+
+def initfft(n):
+    f = np.zeros(5, np.uint64)
+    trig = np.zeros(2 * n)
+    _check(_lib.sw_initfft(C.c_size_t(n), f.ctypes.data_as(_sp), _d(trig)))
+    return [int(x) for x in f], trig
+
+
+def forfft(m, n, x, trig=None, factors=None):
+    """In place: m real transforms of length n of x(m,n) (element (i,j) at i + m*j)."""
+    fa, fp = _sz(factors) if factors is not None else (None, None)
+    _check(_lib.sw_forfft(C.c_size_t(m), C.c_size_t(n), _d(x), None if trig is None else _d(trig), fp))
+
+
+def revfft(m, n, x, trig=None, factors=None):
+    fa, fp = _sz(factors) if factors is not None else (None, None)
+    _check(_lib.sw_revfft(C.c_size_t(m), C.c_size_t(n), _d(x), None if trig is None else _d(trig), fp))
+
+
+# ---------------------------------------------------------------- sta2dfft (src/sta2dfft.rs)
+class D2FFT:
+    def __init__(self, nx, ny, lx=2 * np.pi, ly=2 * np.pi):
+        self.nx, self.ny = nx, ny
+        self.kx, self.ky = np.zeros(nx), np.zeros(ny)
+        h = _vp()
+        _check(_lib.sw_d2fft_create(C.c_size_t(nx), C.c_size_t(ny), C.c_double(lx), C.c_double(ly), _d(self.kx),
+                                    _d(self.ky), C.byref(h)))
+        self._h = h
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            _lib.sw_d2fft_destroy(self._h)
+            self._h = None
+
+    def ptospc(self, rvar, svar):
+        _check(_lib.sw_ptospc(self._h, _d(rvar), _d(svar)))
+
+    def spctop(self, svar, rvar):
+        _check(_lib.sw_spctop(self._h, _d(svar), _d(rvar)))
+
+    def xderiv(self, rkx, var, der):
+        _check(_lib.sw_xderiv(self._h, _d(rkx), _d(var), _d(der)))
+
+    def yderiv(self, rky, var, der):
+        _check(_lib.sw_yderiv(self._h, _d(rky), _d(var), _d(der)))
+
+
+# ---------------------------------------------------------------- spectral (src/spectral/mod.rs)
+class Spectral:
+    def __init__(self, ng, nz, precision=FP64):
+        self.ng, self.nz, self.precision = ng, nz, precision
+        h = _vp()
+        _check(_lib.sw_spectral_create(C.c_size_t(ng), C.c_size_t(nz), C.c_int(precision), C.byref(h)))
+        self._h = h
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            _lib.sw_spectral_destroy(self._h)
+            self._h = None
+
+    def field(self):
+        return np.zeros(self.ng * self.ng * (self.nz + 1))
+
+    def slab(self):
+        return np.zeros(self.ng * self.ng)
+
+    def get(self, name):
+        n = _lib.sw_spectral_get(self._h, name.encode(), None)
+        if n < 0:
+            raise KeyError(name)
+        out = np.zeros(n)
+        _lib.sw_spectral_get(self._h, name.encode(), _d(out))
+        return out
+
+    def main_invert(self, qs, ds, gs, r, u, v, zeta):
+        _check(_lib.sw_main_invert(self._h, _d(qs), _d(ds), _d(gs), _d(r), _d(u), _d(v), _d(zeta)))
+
+    def jacob(self, aa, bb, cs):
+        _check(_lib.sw_jacob(self._h, _d(aa), _d(bb), _d(cs)))
+
+    def divs(self, aa, bb, cs):
+        _check(_lib.sw_divs(self._h, _d(aa), _d(bb), _d(cs)))
+
+    def ptospc3d(self, fp, fs, izbeg, izend):
+        _check(_lib.sw_ptospc3d(self._h, _d(fp), _d(fs), C.c_size_t(izbeg), C.c_size_t(izend)))
+
+    def spctop3d(self, fs, fp, izbeg, izend):
+        _check(_lib.sw_spctop3d(self._h, _d(fs), _d(fp), C.c_size_t(izbeg), C.c_size_t(izend)))
+
+    def deal3d(self, fp):
+        _check(_lib.sw_deal3d(self._h, _d(fp)))
+
+    def deal2d(self, fp):
+        _check(_lib.sw_deal2d(self._h, _d(fp)))
+
+
+# ---------------------------------------------------------------- nhswps (src/nhswps)
+class State:
+    """Device-resident counterpart of the reference State (src/nhswps/mod.rs:87-133)."""
+
+    def __init__(self, spectral, t=0.0, ggen=False, **fields):
+        self.spectral = spectral
+        h = _vp()
+        _check(_lib.sw_state_create(spectral._h, C.byref(h)))
+        self._h = h
+        for name, a in fields.items():
+            self.set(name, a)
+        self.set_time(t, ggen)
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            _lib.sw_state_destroy(self._h)
+            self._h = None
+
+    def set(self, name, a):
+        a = np.ascontiguousarray(a, dtype=np.float64).ravel()
+        _check(_lib.sw_state_set(self._h, C.c_int(FIELDS.index(name)), _d(a)))
+
+    def get(self, name):
+        out = self.spectral.field()
+        _check(_lib.sw_state_get(self._h, C.c_int(FIELDS.index(name)), _d(out)))
+        return out
+
+    def set_time(self, t, ggen=False):
+        _check(_lib.sw_state_set_time(self._h, C.c_double(t), C.c_int(1 if ggen else 0)))
+
+    @property
+    def t(self):
+        t = C.c_double()
+        _check(_lib.sw_state_get_time(self._h, C.byref(t)))
+        return t.value
+
+    def __getattr__(self, name):
+        if name in FIELDS:
+            return self.get(name)
+        raise AttributeError(name)
+
+
+def vertical(st):
+    _check(_lib.sw_vertical(st._h))
+
+
+def coeffs(st, sigx, sigy, cpt1, cpt2):
+    _check(_lib.sw_coeffs(st._h, _d(sigx), _d(sigy), _d(cpt1), _d(cpt2)))
+
+
+def cpsource(st, sp0):
+    _check(_lib.sw_cpsource(st._h, _d(sp0)))
+
+
+def psolve(st, nitmax=100):
+    """Returns (iterations, final errp)."""
+    it, errp = C.c_int(), C.c_double()
+    _check(_lib.sw_psolve(st._h, C.c_int(nitmax), C.byref(it), C.byref(errp)))
+    return it.value, errp.value
+
+
+def source(st, sqs, sds, sgs):
+    _check(_lib.sw_source(st._h, _d(sqs), _d(sds), _d(sgs)))
+
+
+def diagnose(st):
+    out = np.zeros(5)
+    _check(_lib.sw_diagnose(st._h, _d(out)))
+    return out
+
+
+def advance(st):
+    """One timestep; returns (monitor values [t, zrms^2/2, zrms, zmax, umax], psolve iterations)."""
+    diag = np.zeros(5)
+    iters = np.zeros(3, np.int32)
+    _check(_lib.sw_advance(st._h, _d(diag), iters.ctypes.data_as(C.POINTER(C.c_int))))
+    return diag, [int(i) for i in iters]
+
+
+def synth_kminus3(ng, nz, seed, k0=8.0, zeta_rms=0.5 * 4 * np.pi, delta_rms=0.0):
+    """Seeded spectral PV (and divergence) with an isotropic k^-3 spectrum peaked at k0, identical on
+    every level; returns (qs, ds) as spectral fields."""
+    qs = np.zeros(ng * ng * (nz + 1))
+    ds = np.zeros_like(qs)
+    _check(_lib.sw_synth_kminus3(C.c_size_t(ng), C.c_size_t(nz), C.c_ulonglong(seed), C.c_double(k0),
+                                 C.c_double(zeta_rms), C.c_double(delta_rms), _d(qs), _d(ds)))
+    return qs, ds

@@ -1,0 +1,653 @@
+// extern "C" entry points of libsw_hip.so (declared in include/shallow_water.h).  Each entry point
+// validates its arguments, stages host arrays through device memory when needed, dispatches on the
+// precision of the handle and turns exceptions into sw_status codes.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <memory>
+#include <mutex>
+#include <string>
+
+#include "shallow_water.h"
+#include "sw_engine.hpp"
+
+using namespace sw;
+
+namespace {
+
+thread_local std::string g_err;
+int g_device = 0;
+hipStream_t g_stream = nullptr;
+std::mutex g_mu;
+
+hipStream_t stream() {
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (!g_stream) {
+        SW_HIP(hipSetDevice(g_device));
+        SW_HIP(hipStreamCreateWithFlags(&g_stream, hipStreamNonBlocking));
+    }
+    return g_stream;
+}
+
+template <class F>
+int guard(F f) {
+    try {
+        return f();
+    } catch (const Error& e) {
+        g_err = e.what();
+        return e.code;
+    } catch (const std::exception& e) {
+        g_err = e.what();
+        return SW_ERR_ARG;
+    }
+}
+
+__global__ void k_cvt_d2f(const double* a, float* b, size_t n) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) b[i] = (float)a[i];
+}
+__global__ void k_cvt_f2d(const float* a, double* b, size_t n) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) b[i] = (double)a[i];
+}
+
+// One array argument at the boundary: a double* that may be host or device memory, seen by the
+// kernels as a device T*.
+template <typename T>
+struct Arg {
+    double* user;
+    size_t n;
+    bool out;
+    bool dev;
+    DevBuf<double> stage;
+    DevBuf<T> conv;
+    T* d = nullptr;
+    hipStream_t st;
+    Arg(const double* u, size_t count, bool in, bool out_, hipStream_t s)
+        : user(const_cast<double*>(u)), n(count), out(out_), st(s) {
+        SW_CHECK(u != nullptr, SW_ERR_ARG, "null array argument");
+        dev = is_device_ptr(u);
+        double* dd = user;
+        if (!dev) {
+            stage.alloc(n);
+            dd = stage.p;
+            if (in) SW_HIP(hipMemcpyAsync(dd, user, n * sizeof(double), hipMemcpyHostToDevice, st));
+        }
+        if constexpr (sizeof(T) == sizeof(double)) {
+            d = reinterpret_cast<T*>(dd);
+        } else {
+            conv.alloc(n);
+            d = conv.p;
+            if (in) hipLaunchKernelGGL(k_cvt_d2f, dim3((n + 255) / 256), dim3(256), 0, st, dd, conv.p, n);
+        }
+    }
+    void finish() {
+        if (!out) return;
+        double* dd = dev ? user : stage.p;
+        if constexpr (sizeof(T) != sizeof(double))
+            hipLaunchKernelGGL(k_cvt_f2d, dim3((n + 255) / 256), dim3(256), 0, st, conv.p, dd, n);
+        if (!dev) SW_HIP(hipMemcpyAsync(user, dd, n * sizeof(double), hipMemcpyDeviceToHost, st));
+        SW_HIP(hipStreamSynchronize(st));
+    }
+};
+
+}  // namespace
+
+struct sw_d2fft {
+    size_t nx, ny;
+    std::unique_ptr<Engine<double>> e;
+};
+
+struct sw_spectral {
+    int precision;
+    size_t ng, nz;
+    std::unique_ptr<Engine<double>> e64;
+    std::unique_ptr<Engine<float>> e32;
+};
+
+struct sw_state {
+    sw_spectral* s;
+    double t = 0.0;
+    int ggen = 0;
+    DevBuf<double> f64[SW_NFIELDS];
+    DevBuf<float> f32[SW_NFIELDS];
+    template <typename T>
+    typename Engine<T>::Fields fields() {
+        typename Engine<T>::Fields f;
+        for (int i = 0; i < SW_NFIELDS; i++) {
+            if constexpr (sizeof(T) == 8) f.a[i] = f64[i].p;
+            else f.a[i] = f32[i].p;
+        }
+        return f;
+    }
+};
+
+// Run fn(engine) with the engine of the spectral handle's precision.
+template <class F>
+static int with_engine(sw_spectral* s, F fn) {
+    SW_CHECK(s != nullptr, SW_ERR_ARG, "null spectral handle");
+    if (s->precision == SW_FP64) return fn(*s->e64);
+    return fn(*s->e32);
+}
+
+template <class F>
+static int with_state(sw_state* st, F fn) {
+    SW_CHECK(st && st->s, SW_ERR_ARG, "null state");
+    return with_engine(st->s, [&](auto& e) {
+        using T = std::remove_reference_t<decltype(*e.tmp.p)>;
+        return fn(e, st->template fields<T>());
+    });
+}
+
+extern "C" {
+
+const char* sw_last_error(void) { return g_err.c_str(); }
+
+int sw_init(int device) {
+    return guard([&] {
+        std::lock_guard<std::mutex> lk(g_mu);
+        int cnt = 0;
+        SW_HIP(hipGetDeviceCount(&cnt));
+        SW_CHECK(device >= 0 && device < cnt, SW_ERR_ARG, "no such device");
+        g_device = device;
+        SW_HIP(hipSetDevice(device));
+        if (g_stream) {
+            SW_HIP(hipStreamDestroy(g_stream));
+            g_stream = nullptr;
+        }
+        return 0;
+    });
+}
+
+int sw_sync(void) {
+    return guard([&] {
+        SW_HIP(hipStreamSynchronize(stream()));
+        return 0;
+    });
+}
+
+int sw_factorisen(size_t n, size_t factors[5]) {
+    return guard([&] {
+        SW_CHECK(factors, SW_ERR_ARG, "null factors");
+        return factorise_ref(n, factors) ? 0 : (int)SW_ERR_FACTOR;
+    });
+}
+
+int sw_initfft(size_t n, size_t factors[5], double* trig) {
+    return guard([&] {
+        SW_CHECK(factors && trig, SW_ERR_ARG, "null argument");
+        SW_CHECK(factorise_ref(n, factors), SW_ERR_FACTOR, "factorisation failed");
+        initfft_ref(n, factors, trig);
+        return 0;
+    });
+}
+
+static int fft1d(size_t m, size_t n, double* x, const size_t factors[5], bool inv) {
+    return guard([&] {
+        SW_CHECK(x && m > 0 && n >= 2 && m <= (1u << 30) && n <= (1u << 16), SW_ERR_ARG, "bad forfft/revfft shape");
+        size_t f[5];
+        SW_CHECK(factorise_ref(n, f), SW_ERR_FACTOR, "factorisation failed");
+        if (factors) SW_CHECK(std::memcmp(f, factors, sizeof(f)) == 0, SW_ERR_ARG, "factors do not match n");
+        SW_CHECK(reference_transform_is_dft(f), SW_ERR_UNSUPPORTED, "length not supported");
+        hipStream_t st = stream();
+        FftPlan<double> pl{};
+        DevBuf<C2<double>> tw;
+        size_t lds;
+        // rows (the batch index i of x(m,n)) per block must divide m
+        int B = 1;
+        for (int b = 16; b >= 2; b /= 2)
+            if (m % b == 0 && (size_t)b * n * sizeof(double) <= 65536 && (size_t)(b / 2) * n <= (size_t)FFT_EPT_MAX * 512) {
+                B = b;
+                break;
+            }
+        Engine<double>::make_plan((int)n, pl, tw, lds, B, 0);
+        pl.rows = (int)m;
+        pl.xcd = 0;
+        Arg<double> a(x, m * n, true, true, st);
+        double* d = a.d;
+        const size_t mm = m;
+        auto pro = [=] __device__(int, int r, int j) { return d[(size_t)r + mm * j]; };
+        auto epi = [=] __device__(int, int r, int f, double v) { d[(size_t)r + mm * f] = v; };
+        dim3 grid((unsigned)(m / B)), block(pl.nt);
+        if (!inv)
+            hipLaunchKernelGGL((fft_fwd_pass<double, false, decltype(pro), decltype(epi)>), grid, block, lds, st, pl, 1, pro,
+                               epi, (double*)nullptr);
+        else
+            hipLaunchKernelGGL((fft_inv_pass<double, false, decltype(pro), decltype(epi)>), grid, block, lds, st, pl, 1, pro,
+                               epi, (double*)nullptr);
+        SW_HIP(hipGetLastError());
+        a.finish();
+        return 0;
+    });
+}
+
+int sw_forfft(size_t m, size_t n, double* x, const double* trig, const size_t factors[5]) {
+    (void)trig;
+    return fft1d(m, n, x, factors, false);
+}
+int sw_revfft(size_t m, size_t n, double* x, const double* trig, const size_t factors[5]) {
+    (void)trig;
+    return fft1d(m, n, x, factors, true);
+}
+
+int sw_d2fft_create(size_t nx, size_t ny, double lx, double ly, double* kx, double* ky, sw_d2fft** out) {
+    return guard([&] {
+        SW_CHECK(out, SW_ERR_ARG, "null output handle");
+        SW_CHECK(nx == ny, SW_ERR_UNSUPPORTED, "only square grids (nx == ny) are supported");
+        SW_CHECK(lx != 0.0, SW_ERR_ARG, "domain length in x equal to zero not allowed");
+        auto d = std::make_unique<sw_d2fft>();
+        d->nx = nx;
+        d->ny = ny;
+        d->e = std::make_unique<Engine<double>>((int)nx, 2, stream());
+        if (kx) for (size_t k = 1; k <= nx; k++) kx[k - 1] = (PI / lx) * (double)k;
+        if (ky && ly != 0.0) for (size_t k = 1; k <= ny; k++) ky[k - 1] = (PI / ly) * (double)k;
+        *out = d.release();
+        return 0;
+    });
+}
+
+int sw_d2fft_destroy(sw_d2fft* d) {
+    delete d;
+    return 0;
+}
+
+int sw_ptospc(const sw_d2fft* d, double* rvar, double* svar) {
+    return guard([&] {
+        SW_CHECK(d, SW_ERR_ARG, "null handle");
+        auto& e = *d->e;
+        Arg<double> a(rvar, e.n2, true, false, e.st), b(svar, e.n2, false, true, e.st);
+        e.fwd2d(1, e.ld(a.d), e.stp(b.d));
+        b.finish();
+        return 0;
+    });
+}
+
+int sw_spctop(const sw_d2fft* d, double* svar, double* rvar) {
+    return guard([&] {
+        SW_CHECK(d, SW_ERR_ARG, "null handle");
+        auto& e = *d->e;
+        Arg<double> a(svar, e.n2, true, false, e.st), b(rvar, e.n2, false, true, e.st);
+        e.inv2d(1, e.ld(a.d), e.stp(b.d));
+        b.finish();
+        return 0;
+    });
+}
+
+// xderiv/yderiv with the caller's wavenumber array, exactly as sta2dfft.rs:128-184
+static int deriv(const sw_d2fft* d, const double* rk, const double* var, double* der, bool ydir) {
+    return guard([&] {
+        SW_CHECK(d && rk, SW_ERR_ARG, "null argument");
+        auto& e = *d->e;
+        const int n = (int)d->nx;
+        Arg<double> k(rk, (size_t)n, true, false, e.st), a(var, e.n2, true, false, e.st), b(der, e.n2, false, true, e.st);
+        const double *K = k.d, *A = a.d;
+        double* Dd = b.d;
+        each(e.st, e.n2, [=] __device__(size_t p) {
+            int kx = p % n, ky = p / n;
+            int i = ydir ? ky : kx;
+            double val = 0.0;
+            if (i != 0 && 2 * i != n) {
+                size_t q = ydir ? kx + (size_t)n * (n - ky) : (n - kx) + (size_t)n * ky;
+                val = 2 * i < n ? -K[2 * i - 1] * A[q] : K[2 * (n - i) - 1] * A[q];
+            }
+            Dd[p] = val;
+        });
+        b.finish();
+        return 0;
+    });
+}
+int sw_xderiv(const sw_d2fft* d, const double* rkx, const double* var, double* der) { return deriv(d, rkx, var, der, false); }
+int sw_yderiv(const sw_d2fft* d, const double* rky, const double* var, double* der) { return deriv(d, rky, var, der, true); }
+
+int sw_spectral_create(size_t ng, size_t nz, int precision, sw_spectral** out) {
+    return guard([&] {
+        SW_CHECK(out, SW_ERR_ARG, "null output handle");
+        SW_CHECK(precision == SW_FP64 || precision == SW_MIXED, SW_ERR_ARG, "bad precision");
+        auto s = std::make_unique<sw_spectral>();
+        s->precision = precision;
+        s->ng = ng;
+        s->nz = nz;
+        if (precision == SW_FP64) s->e64 = std::make_unique<Engine<double>>((int)ng, (int)nz, stream());
+        else s->e32 = std::make_unique<Engine<float>>((int)ng, (int)nz, stream());
+        *out = s.release();
+        return 0;
+    });
+}
+
+int sw_spectral_destroy(sw_spectral* s) {
+    delete s;
+    return 0;
+}
+
+long sw_spectral_get(const sw_spectral* s, const char* name, double* out) {
+    try {
+        SW_CHECK(s && name, SW_ERR_ARG, "null argument");
+        const HostOps& h = s->precision == SW_FP64 ? s->e64->host : s->e32->host;
+        std::string n(name);
+        const std::vector<double>* v = nullptr;
+        if (n == "hlap") v = &h.hlap; else if (n == "glap") v = &h.glap; else if (n == "rlap") v = &h.rlap;
+        else if (n == "helm") v = &h.helm; else if (n == "c2g2") v = &h.c2g2; else if (n == "simp") v = &h.simp;
+        else if (n == "rope") v = &h.rope; else if (n == "fope") v = &h.fope; else if (n == "filt") v = &h.filt;
+        else if (n == "diss") v = &h.diss; else if (n == "opak") v = &h.opak; else if (n == "rdis") v = &h.rdis;
+        else if (n == "etdv") v = &h.etdv; else if (n == "htdv") v = &h.htdv; else if (n == "ap") v = &h.ap;
+        else if (n == "etd1") v = &h.etd1; else if (n == "htd1") v = &h.htd1; else if (n == "theta") v = &h.theta;
+        else if (n == "weight") v = &h.weight; else if (n == "hrkx") v = &h.hrkx; else if (n == "hrky") v = &h.hrky;
+        else if (n == "rk") v = &h.rk; else if (n == "spmf") v = &h.spmf; else if (n == "alk") v = &h.alk;
+        else if (n == "kmag") v = &h.kmag; else if (n == "xtrig" || n == "ytrig") v = &h.xtrig;
+        else if (n == "kmax" || n == "kmaxred") {
+            if (out) out[0] = (double)(n == "kmax" ? h.kmax : h.kmaxred);
+            return 1;
+        } else if (n == "xfactors" || n == "yfactors") {
+            if (out) for (int i = 0; i < 5; i++) out[i] = (double)h.xf[i];
+            return 5;
+        }
+        SW_CHECK(v != nullptr, SW_ERR_ARG, "unknown operator name");
+        if (out) std::memcpy(out, v->data(), v->size() * sizeof(double));
+        return (long)v->size();
+    } catch (const Error& e) {
+        g_err = e.what();
+        return e.code;
+    }
+}
+
+int sw_main_invert(sw_spectral* s, const double* qs, const double* ds, const double* gs, double* r, double* u, double* v,
+                   double* zeta) {
+    return guard([&] {
+        return with_engine(s, [&](auto& e) {
+            using T = std::remove_reference_t<decltype(*e.tmp.p)>;
+            Arg<T> aq(qs, e.N, true, false, e.st), ad(ds, e.N, true, false, e.st), ag(gs, e.N, true, false, e.st);
+            Arg<T> ar(r, e.N, false, true, e.st), au(u, e.N, false, true, e.st), av(v, e.N, false, true, e.st),
+                az(zeta, e.N, false, true, e.st);
+            e.main_invert(aq.d, ad.d, ag.d, ar.d, au.d, av.d, az.d);
+            ar.finish(); au.finish(); av.finish(); az.finish();
+            return 0;
+        });
+    });
+}
+
+int sw_jacob(sw_spectral* s, const double* aa, const double* bb, double* cs) {
+    return guard([&] {
+        return with_engine(s, [&](auto& e) {
+            using T = std::remove_reference_t<decltype(*e.tmp.p)>;
+            Arg<T> a(aa, e.n2, true, false, e.st), b(bb, e.n2, true, false, e.st), c(cs, e.n2, false, true, e.st);
+            e.jacob(a.d, b.d, c.d);
+            c.finish();
+            return 0;
+        });
+    });
+}
+
+int sw_divs(sw_spectral* s, const double* aa, const double* bb, double* cs) {
+    return guard([&] {
+        return with_engine(s, [&](auto& e) {
+            using T = std::remove_reference_t<decltype(*e.tmp.p)>;
+            Arg<T> a(aa, e.n2, true, false, e.st), b(bb, e.n2, true, false, e.st), c(cs, e.n2, false, true, e.st);
+            e.divs(a.d, b.d, c.d);
+            c.finish();
+            return 0;
+        });
+    });
+}
+
+static int p3d(sw_spectral* s, const double* in, double* out, size_t izbeg, size_t izend, bool fwd) {
+    return guard([&] {
+        SW_CHECK(s && izbeg <= izend && izend <= s->nz, SW_ERR_ARG, "bad level range");
+        return with_engine(s, [&](auto& e) {
+            using T = std::remove_reference_t<decltype(*e.tmp.p)>;
+            // the output's untouched levels must survive: read it in too
+            Arg<T> a(in, e.N, true, false, e.st), b(out, e.N, true, true, e.st);
+            if (fwd) e.ptospc3d(a.d, b.d, (int)izbeg, (int)izend);
+            else e.spctop3d(a.d, b.d, (int)izbeg, (int)izend);
+            b.finish();
+            return 0;
+        });
+    });
+}
+int sw_ptospc3d(sw_spectral* s, const double* fp, double* fs, size_t izbeg, size_t izend) { return p3d(s, fp, fs, izbeg, izend, true); }
+int sw_spctop3d(sw_spectral* s, const double* fs, double* fp, size_t izbeg, size_t izend) { return p3d(s, fs, fp, izbeg, izend, false); }
+
+static int deal_n(sw_spectral* s, double* fp, bool three) {
+    return guard([&] {
+        return with_engine(s, [&](auto& e) {
+            using T = std::remove_reference_t<decltype(*e.tmp.p)>;
+            size_t cnt = three ? e.N : e.n2;
+            Arg<T> a(fp, cnt, true, true, e.st);
+            e.deal(a.d, three ? e.L : 1);
+            a.finish();
+            return 0;
+        });
+    });
+}
+int sw_deal3d(sw_spectral* s, double* fp) { return deal_n(s, fp, true); }
+int sw_deal2d(sw_spectral* s, double* fp) { return deal_n(s, fp, false); }
+
+int sw_state_create(sw_spectral* s, sw_state** out) {
+    return guard([&] {
+        SW_CHECK(s && out, SW_ERR_ARG, "null argument");
+        auto st = std::make_unique<sw_state>();
+        st->s = s;
+        size_t N = s->ng * s->ng * (s->nz + 1);
+        hipStream_t hs = stream();
+        for (int i = 0; i < SW_NFIELDS; i++) {
+            if (s->precision == SW_FP64) {
+                st->f64[i].alloc(N);
+                SW_HIP(hipMemsetAsync(st->f64[i].p, 0, N * sizeof(double), hs));
+            } else {
+                st->f32[i].alloc(N);
+                SW_HIP(hipMemsetAsync(st->f32[i].p, 0, N * sizeof(float), hs));
+            }
+        }
+        SW_HIP(hipStreamSynchronize(hs));
+        *out = st.release();
+        return 0;
+    });
+}
+
+int sw_state_destroy(sw_state* st) {
+    delete st;
+    return 0;
+}
+
+int sw_state_set(sw_state* st, int field, const double* src) {
+    return guard([&] {
+        SW_CHECK(field >= 0 && field < SW_NFIELDS, SW_ERR_ARG, "bad field id");
+        return with_state(st, [&](auto& e, auto f) {
+            using T = std::remove_reference_t<decltype(*e.tmp.p)>;
+            Arg<T> a(src, e.N, true, false, e.st);
+            SW_HIP(hipMemcpyAsync(f.a[field], a.d, e.N * sizeof(T), hipMemcpyDeviceToDevice, e.st));
+            SW_HIP(hipStreamSynchronize(e.st));
+            return 0;
+        });
+    });
+}
+
+int sw_state_get(sw_state* st, int field, double* dst) {
+    return guard([&] {
+        SW_CHECK(field >= 0 && field < SW_NFIELDS, SW_ERR_ARG, "bad field id");
+        return with_state(st, [&](auto& e, auto f) {
+            using T = std::remove_reference_t<decltype(*e.tmp.p)>;
+            Arg<T> a(dst, e.N, false, true, e.st);
+            SW_HIP(hipMemcpyAsync(a.d, f.a[field], e.N * sizeof(T), hipMemcpyDeviceToDevice, e.st));
+            a.finish();
+            return 0;
+        });
+    });
+}
+
+int sw_state_set_time(sw_state* st, double t, int ggen) {
+    return guard([&] {
+        SW_CHECK(st, SW_ERR_ARG, "null state");
+        st->t = t;
+        st->ggen = ggen;
+        return 0;
+    });
+}
+
+int sw_state_get_time(const sw_state* st, double* t) {
+    return guard([&] {
+        SW_CHECK(st && t, SW_ERR_ARG, "null argument");
+        *t = st->t;
+        return 0;
+    });
+}
+
+int sw_vertical(sw_state* st) {
+    return guard([&] {
+        return with_state(st, [&](auto& e, auto f) {
+            e.vertical(f);
+            SW_HIP(hipStreamSynchronize(e.st));
+            return 0;
+        });
+    });
+}
+
+int sw_coeffs(sw_state* st, double* sigx, double* sigy, double* cpt1, double* cpt2) {
+    return guard([&] {
+        return with_state(st, [&](auto& e, auto f) {
+            using T = std::remove_reference_t<decltype(*e.tmp.p)>;
+            Arg<T> a(sigx, e.N, false, true, e.st), b(sigy, e.N, false, true, e.st), c(cpt1, e.N, false, true, e.st),
+                d(cpt2, e.N, false, true, e.st);
+            e.coeffs(f, a.d, b.d, c.d, d.d);
+            a.finish(); b.finish(); c.finish(); d.finish();
+            return 0;
+        });
+    });
+}
+
+int sw_cpsource(sw_state* st, double* sp0) {
+    return guard([&] {
+        return with_state(st, [&](auto& e, auto f) {
+            using T = std::remove_reference_t<decltype(*e.tmp.p)>;
+            Arg<T> a(sp0, e.N, false, true, e.st);
+            e.cpsource(f, a.d);
+            a.finish();
+            return 0;
+        });
+    });
+}
+
+int sw_psolve(sw_state* st, int nitmax, int* iters, double* errp) {
+    return guard([&] {
+        SW_CHECK(nitmax >= 1, SW_ERR_ARG, "nitmax must be >= 1");
+        SW_CHECK(st && st->s && st->s->nz >= 3, SW_ERR_ARG, "psolve needs nz >= 3");
+        return with_state(st, [&](auto& e, auto f) {
+            int rc = e.psolve(f, nitmax, iters, errp);
+            SW_HIP(hipStreamSynchronize(e.st));
+            return rc;
+        });
+    });
+}
+
+int sw_source(sw_state* st, double* sqs, double* sds, double* sgs) {
+    return guard([&] {
+        return with_state(st, [&](auto& e, auto f) {
+            using T = std::remove_reference_t<decltype(*e.tmp.p)>;
+            Arg<T> a(sqs, e.N, false, true, e.st), b(sds, e.N, false, true, e.st), c(sgs, e.N, false, true, e.st);
+            e.source(f, a.d, b.d, c.d);
+            a.finish(); b.finish(); c.finish();
+            return 0;
+        });
+    });
+}
+
+int sw_diagnose(sw_state* st, double diag[5]) {
+    return guard([&] {
+        SW_CHECK(diag, SW_ERR_ARG, "null diag");
+        return with_state(st, [&](auto& e, auto f) {
+            e.diagnose(f, st->t, diag);
+            return 0;
+        });
+    });
+}
+
+int sw_advance(sw_state* st, double diag[5], int iters[3]) {
+    return guard([&] {
+        SW_CHECK(st && st->s && st->s->nz >= 3, SW_ERR_ARG, "advance needs nz >= 3");
+        double dg[5];
+        int it[3];
+        int rc = with_state(st, [&](auto& e, auto f) {
+            int r = e.advance(f, st->t, st->ggen != 0, dg, it);
+            SW_HIP(hipStreamSynchronize(e.st));
+            return r;
+        });
+        if (diag) std::memcpy(diag, dg, sizeof(dg));
+        if (iters) std::memcpy(iters, it, sizeof(it));
+        return rc;
+    });
+}
+
+// ---------------------------------------------------------------- synthetic initial state
+// splitmix64 seeding xoshiro256**, Box-Muller normals.
+namespace {
+struct Rng {
+    uint64_t s[4];
+    explicit Rng(uint64_t seed) {
+        for (int i = 0; i < 4; i++) {
+            seed += 0x9E3779B97F4A7C15ull;
+            uint64_t z = seed;
+            z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+            z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+            s[i] = z ^ (z >> 31);
+        }
+    }
+    static uint64_t rotl(uint64_t x, int k) { return (x << k) | (x >> (64 - k)); }
+    uint64_t next() {
+        uint64_t r = rotl(s[1] * 5, 7) * 9, t = s[1] << 17;
+        s[2] ^= s[0]; s[3] ^= s[1]; s[1] ^= s[2]; s[0] ^= s[3]; s[2] ^= t; s[3] = rotl(s[3], 45);
+        return r;
+    }
+    double uniform() { return ((next() >> 11) + 0.5) * (1.0 / 9007199254740992.0); }
+    double normal() {
+        double u1 = uniform(), u2 = uniform();
+        return std::sqrt(-2.0 * std::log(u1)) * std::cos(2.0 * PI * u2);
+    }
+};
+
+// one spectral slab with an isotropic k^3/(k0^6+k^6)/k power law, de-aliased, zero mean, scaled so
+// that the physical rms of op(k)*field equals rms
+void synth_slab(const HostOps& h, Rng& g, double k0, double rms, const std::vector<double>* op, std::vector<double>& out) {
+    size_t ng = h.ng, n2 = ng * ng;
+    out.assign(n2, 0.0);
+    double ss = 0.0;
+    for (size_t ky = 0; ky < ng; ky++)
+        for (size_t kx = 0; kx < ng; kx++) {
+            size_t p = kx + ng * ky;
+            double k = std::sqrt(h.rk[kx] * h.rk[kx] + h.rk[ky] * h.rk[ky]);
+            double x = g.normal();
+            if (k == 0.0 || h.filt[p] == 0.0) continue;
+            double spec = k * k * k / (std::pow(k0, 6.0) + std::pow(k, 6.0));
+            out[p] = std::sqrt(spec / k) * x;
+            double wx = (kx == 0 || 2 * kx == ng) ? 1.0 : 2.0, wy = (ky == 0 || 2 * ky == ng) ? 1.0 : 2.0;
+            double y = op ? (*op)[p] * out[p] : out[p];
+            ss += wx * wy * y * y;
+        }
+    double cur = std::sqrt(ss / (double)n2);
+    double a = cur > 0.0 ? rms / cur : 0.0;
+    for (auto& x : out) x *= a;
+}
+}  // namespace
+
+int sw_synth_kminus3(size_t ng, size_t nz, unsigned long long seed, double k0, double zeta_rms, double delta_rms, double* qq,
+                     double* dd) {
+    return guard([&] {
+        SW_CHECK(qq && dd && nz >= 1 && ng >= 4, SW_ERR_ARG, "bad argument");
+        HostOps h;
+        SW_CHECK(h.build(ng, nz < 2 ? 2 : nz), SW_ERR_FACTOR, "ng not factorisable");
+        Rng g(seed);
+        std::vector<double> q, d;
+        // zeta_hat = rope * q_hat for identical layers with delta = gamma = 0
+        synth_slab(h, g, k0, zeta_rms, &h.rope, q);
+        if (delta_rms > 0.0) synth_slab(h, g, k0, delta_rms, nullptr, d);
+        else d.assign(ng * ng, 0.0);
+        size_t n2 = ng * ng;
+        for (size_t iz = 0; iz <= nz; iz++) {
+            std::memcpy(qq + iz * n2, q.data(), n2 * sizeof(double));
+            std::memcpy(dd + iz * n2, d.data(), n2 * sizeof(double));
+        }
+        return 0;
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,333 @@
+// The device engine: one per (ng, nz, precision).  Holds the FFT plan, the spectral operators in
+// device memory and the work buffers, and implements the reference's spectral and nhswps
+// functions as batched, fused HIP launches on one stream.  T is the bulk arithmetic type (double,
+// or float in the mixed-precision mode); psolve's tridiagonal solves, its residual and every
+// reduction are carried out in double whatever T is.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <map>
+#include <string>
+
+#include "sw_common.hpp"
+#include "sw_fft.hpp"
+#include "sw_host.hpp"
+
+namespace sw {
+
+// spectral derivative along one index: d[k] = dmul(k) * g[mir(k)] (src/sta2dfft.rs:128-184 with
+// the wavenumbers of D2FFT::new for a 2 pi domain, which are exact integers)
+__device__ __forceinline__ int mir(int k, int n) { return k == 0 ? 0 : n - k; }
+template <typename T>
+__device__ __forceinline__ T dmul(int k, int n) {
+    return (k == 0 || 2 * k == n) ? T(0) : (2 * k < n ? T(-k) : T(n - k));
+}
+
+constexpr int RED_BLOCKS = 512;
+constexpr int MAX_NZ = 256;
+
+template <int K>
+struct Acc {
+    double v[K];
+};
+
+// Block partials of K quantities over i < n; component c is a sum unless bit c of MAXMASK is set,
+// in which case it is a max.  Partials land in part[blockIdx.x * K + c].
+template <int K, unsigned MAXMASK, class F>
+__global__ void __launch_bounds__(256) k_reduce(size_t n, F f, double* part) {
+    __shared__ double sh[256 / 64][K];
+    double acc[K], cmp[K];
+#pragma unroll
+    for (int c = 0; c < K; c++) acc[c] = (MAXMASK >> c) & 1 ? -INFINITY : 0.0, cmp[c] = 0.0;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        Acc<K> a = f(i);
+#pragma unroll
+        for (int c = 0; c < K; c++) {
+            if ((MAXMASK >> c) & 1) {
+                acc[c] = fmax(acc[c], a.v[c]);
+            } else {  // Kahan-compensated sum
+                double y = a.v[c] - cmp[c];
+                double t = acc[c] + y;
+                cmp[c] = (t - acc[c]) - y;
+                acc[c] = t;
+            }
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < K; c++) {
+        double x = acc[c];
+        for (int off = 32; off > 0; off >>= 1) {
+            double y = __shfl_down(x, off, 64);
+            x = ((MAXMASK >> c) & 1) ? fmax(x, y) : x + y;
+        }
+        if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6][c] = x;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int c = 0; c < K; c++) {
+            double x = sh[0][c];
+            for (int w = 1; w < (int)(blockDim.x >> 6); w++) x = ((MAXMASK >> c) & 1) ? fmax(x, sh[w][c]) : x + sh[w][c];
+            part[blockIdx.x * K + c] = x;
+        }
+    }
+}
+
+// Per-level partial sums of two products over the points of each level: grid (nb, nlev).
+template <class F>
+__global__ void __launch_bounds__(256) k_level_sum2(size_t n2, F f, double* part) {
+    __shared__ double sh[4][2];
+    const int iz = blockIdx.y, nb = gridDim.x;
+    double a0 = 0.0, a1 = 0.0;
+    for (size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x; p < n2; p += (size_t)nb * blockDim.x) {
+        Acc<2> v = f(iz, p);
+        a0 += v.v[0];
+        a1 += v.v[1];
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        a0 += __shfl_down(a0, off, 64);
+        a1 += __shfl_down(a1, off, 64);
+    }
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6][0] = a0, sh[threadIdx.x >> 6][1] = a1;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double x0 = sh[0][0], x1 = sh[0][1];
+        for (int w = 1; w < (int)(blockDim.x >> 6); w++) x0 += sh[w][0], x1 += sh[w][1];
+        part[((size_t)iz * nb + blockIdx.x) * 2] = x0;
+        part[((size_t)iz * nb + blockIdx.x) * 2 + 1] = x1;
+    }
+}
+
+// Mean flow of main_invert (spectral/mod.rs:409-416): out[c] = -sum_iz w_iz * S_iz,c / ng^2, with
+// the level sums S taken in a fixed order from the partials (deterministic).
+__global__ void k_meanflow_finalize(const double* part, int nb, int nlev, const double* w, double dsumi, double* out) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    double uio = 0.0, vio = 0.0;
+    for (int iz = 0; iz < nlev; iz++) {
+        double s0 = 0.0, s1 = 0.0;
+        for (int b = 0; b < nb; b++) {
+            s0 += part[((size_t)iz * nb + b) * 2];
+            s1 += part[((size_t)iz * nb + b) * 2 + 1];
+        }
+        uio -= w[iz] * s0 * dsumi;
+        vio -= w[iz] * s1 * dsumi;
+    }
+    out[0] = uio;
+    out[1] = vio;
+}
+
+template <typename T>
+struct Engine {
+    int ng = 0, nz = 0, L = 0;
+    size_t n2 = 0, N = 0;
+    bool mixed = false;
+    hipStream_t st = nullptr;
+    HostOps host;
+    FftPlan<T> plan{};
+    size_t lds_bytes = 0;
+    DevBuf<C2<T>> tw;
+    DevBuf<T> hlap, glap, rlap, c2g2, simp, rope, fope, filt, diss, rdis;
+    DevBuf<double> ap, htdv, etdv;  // FP64 always: the pressure tridiagonal
+    double weight[MAX_NZ + 1], theta[MAX_NZ + 1], htd1[MAX_NZ], etd1[MAX_NZ];
+    DevBuf<double> vc;  // device copy: weight[0..L), theta[L..2L), htd1[2L..2L+nz), etd1[2L+nz..2L+2nz)
+    DevBuf<T> tmp;  // intermediate of the two FFT passes
+    DevBuf<double> part, scal, dwork;
+    std::map<std::string, DevBuf<T>> work;
+    // psolve controls
+    double toler = 1.0e-9;
+    int last_iters = 0;
+    double last_errp = 0.0;
+
+    Engine(int ng_, int nz_, hipStream_t s) : ng(ng_), nz(nz_), L(nz_ + 1), st(s) {
+        mixed = sizeof(T) == 4;
+        n2 = (size_t)ng * ng;
+        N = n2 * L;
+        SW_CHECK(ng >= 4 && nz >= 2 && nz <= MAX_NZ, SW_ERR_ARG, "need ng >= 4 and 2 <= nz <= 256");
+        SW_CHECK(host.build(ng, nz), SW_ERR_FACTOR, "ng is not a product of 2, 3, 4, 5, 6");
+        SW_CHECK(reference_transform_is_dft(host.xf), SW_ERR_UNSUPPORTED,
+                 "ng factorises into stages the reference cannot transform (radix 2 after 5, or 25 | ng)");
+        make_plan(ng, plan, tw, lds_bytes);
+        auto up = [&](DevBuf<T>& d, const std::vector<double>& h) {
+            std::vector<T> c(h.begin(), h.end());
+            d.alloc(c.size());
+            SW_HIP(hipMemcpy(d.p, c.data(), c.size() * sizeof(T), hipMemcpyHostToDevice));
+        };
+        auto upd = [&](DevBuf<double>& d, const std::vector<double>& h) {
+            d.alloc(h.size());
+            SW_HIP(hipMemcpy(d.p, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice));
+        };
+        up(hlap, host.hlap); up(glap, host.glap); up(rlap, host.rlap); up(c2g2, host.c2g2);
+        up(simp, host.simp); up(rope, host.rope); up(fope, host.fope); up(filt, host.filt);
+        up(diss, host.diss); up(rdis, host.rdis);
+        upd(ap, host.ap); upd(htdv, host.htdv); upd(etdv, host.etdv);
+        for (int i = 0; i <= nz; i++) weight[i] = host.weight[i], theta[i] = host.theta[i];
+        for (int i = 0; i < nz; i++) htd1[i] = host.htd1[i], etd1[i] = host.etd1[i];
+        {
+            std::vector<double> h(2 * L + 2 * nz);
+            for (int i = 0; i < L; i++) h[i] = weight[i], h[L + i] = theta[i];
+            for (int i = 0; i < nz; i++) h[2 * L + i] = htd1[i], h[2 * L + nz + i] = etd1[i];
+            upd(vc, h);
+        }
+        part.alloc((size_t)RED_BLOCKS * 4 * (MAX_NZ + 1));
+        scal.alloc(16);
+        if (mixed) toler = 1.0e-6;
+    }
+
+    // ---------------------------------------------------------------- FFT plumbing
+    static void make_plan(int n, FftPlan<T>& pl, DevBuf<C2<T>>& twb, size_t& lds, int force_B = 0, int force_nt = 0) {
+        std::vector<int> r = gpu_radices(n);
+        SW_CHECK(!r.empty() && (int)r.size() <= FFT_MAX_RADICES, SW_ERR_FACTOR, "unsupported transform length");
+        pl.n = n;
+        pl.rows = n;
+        pl.nrad = (int)r.size();
+        for (int i = 0; i < pl.nrad; i++) pl.rad[i] = r[i];
+        const size_t lds_cap = 65536;
+        int B = 1;
+        for (int b = 16; b >= 2; b /= 2)
+            if (n % b == 0 && (size_t)b * n * sizeof(T) <= lds_cap && (b / 2) * n <= FFT_EPT_MAX * 512) {
+                B = b;
+                break;
+            }
+        if (force_B) B = force_B;
+        int npair = B > 1 ? B / 2 : 1;
+        int nt = ((npair * n + FFT_EPT_MAX - 1) / FFT_EPT_MAX + 63) / 64 * 64;
+        nt = std::max(64, std::min(512, nt));
+        if (force_nt) nt = force_nt;
+        SW_CHECK(npair * n <= FFT_EPT_MAX * nt, SW_ERR_ARG, "FFT tile too large for the thread count");
+        pl.B = B;
+        pl.nt = nt;
+        pl.xcd = 1;
+        lds = (size_t)std::max(B, 2) * n * sizeof(T);
+        std::vector<C2<T>> h(n);
+        for (int k = 0; k < n; k++) {
+            double a = -2.0 * PI * (double)k / (double)n;
+            h[k] = C2<T>{T(std::cos(a)), T(std::sin(a))};
+        }
+        twb.alloc(n);
+        SW_HIP(hipMemcpy(twb.p, h.data(), n * sizeof(C2<T>), hipMemcpyHostToDevice));
+        pl.tw = twb.p;
+    }
+
+    T* W(const std::string& name, size_t count) {
+        auto& b = work[name];
+        if (b.n < count) b.alloc(count);
+        return b.p;
+    }
+    T* tmp_for(int nslab) {
+        size_t need = (size_t)nslab * n2;
+        if (tmp.n < need) tmp.alloc(need);
+        return tmp.p;
+    }
+
+    struct NoEpi {
+        __device__ void operator()(int, int, int, T) const {}
+    };
+
+    // Forward 2-D transform of nslab slabs: pro(s, x, y) gives the physical value at (y, x);
+    // epi(s, ky, kx, v) receives the spectral coefficient at (kx, ky).
+    template <class Pro, class Epi>
+    void fwd2d(int nslab, Pro pro, Epi epi) {
+        if (nslab <= 0) return;
+        T* t = tmp_for(nslab);
+        dim3 grid(nslab * (ng / plan.B)), block(plan.nt);
+        hipLaunchKernelGGL((fft_fwd_pass<T, true, Pro, NoEpi>), grid, block, lds_bytes, st, plan, nslab, pro, NoEpi{}, t);
+        SW_HIP(hipGetLastError());
+        LoadPlain<T> lp{t, n2, ng};
+        hipLaunchKernelGGL((fft_fwd_pass<T, false, LoadPlain<T>, Epi>), grid, block, lds_bytes, st, plan, nslab, lp, epi,
+                           (T*)nullptr);
+        SW_HIP(hipGetLastError());
+    }
+
+    // Inverse 2-D transform: pro(s, ky, kx) gives the spectral coefficient at (kx, ky);
+    // epi(s, x, y, v) receives the physical value at (y, x).
+    template <class Pro, class Epi>
+    void inv2d(int nslab, Pro pro, Epi epi) {
+        if (nslab <= 0) return;
+        T* t = tmp_for(nslab);
+        dim3 grid(nslab * (ng / plan.B)), block(plan.nt);
+        hipLaunchKernelGGL((fft_inv_pass<T, true, Pro, NoEpi>), grid, block, lds_bytes, st, plan, nslab, pro, NoEpi{}, t);
+        SW_HIP(hipGetLastError());
+        LoadPlain<T> lp{t, n2, ng};
+        hipLaunchKernelGGL((fft_inv_pass<T, false, LoadPlain<T>, Epi>), grid, block, lds_bytes, st, plan, nslab, lp, epi,
+                           (T*)nullptr);
+        SW_HIP(hipGetLastError());
+    }
+
+    // plain slab accessors (level offset lev0)
+    LoadPlain<T> ld(const T* a, int lev0 = 0) const { return LoadPlain<T>{a + (size_t)lev0 * n2, n2, ng}; }
+    StorePlain<T> stp(T* a, int lev0 = 0) const { return StorePlain<T>{a + (size_t)lev0 * n2, n2, ng}; }
+
+    template <int K, unsigned MAXMASK, class F>
+    void reduce(size_t n, F f, double* out_host) {
+        int blocks = (int)std::min<size_t>(RED_BLOCKS, (n + 255) / 256);
+        hipLaunchKernelGGL((k_reduce<K, MAXMASK, F>), dim3(blocks), dim3(256), 0, st, n, f, part.p);
+        SW_HIP(hipGetLastError());
+        std::vector<double> h((size_t)blocks * K);
+        SW_HIP(hipMemcpyAsync(h.data(), part.p, h.size() * sizeof(double), hipMemcpyDeviceToHost, st));
+        SW_HIP(hipStreamSynchronize(st));
+        for (int c = 0; c < K; c++) {
+            bool mx = (MAXMASK >> c) & 1;
+            double acc = mx ? -INFINITY : 0.0, cmp = 0.0;
+            for (int b = 0; b < blocks; b++) {
+                double x = h[(size_t)b * K + c];
+                if (mx) {
+                    acc = std::fmax(acc, x);
+                } else {
+                    double y = x - cmp, t = acc + y;
+                    cmp = (t - acc) - y;
+                    acc = t;
+                }
+            }
+            out_host[c] = acc;
+        }
+    }
+
+    // ---------------------------------------------------------------- spectral module
+    void ptospc3d(const T* fp, T* fs, int izbeg, int izend) { fwd2d(izend - izbeg + 1, ld(fp, izbeg), stp(fs, izbeg)); }
+    void spctop3d(const T* fs, T* fp, int izbeg, int izend) { inv2d(izend - izbeg + 1, ld(fs, izbeg), stp(fp, izbeg)); }
+
+    // deal3d / deal2d: ptospc, multiply by the 2/3-rule filter, spctop (spectral/mod.rs:524-554)
+    void deal(T* fp, int nlev) {
+        T* S = W("deal", (size_t)nlev * n2);
+        const T* F = filt.p;
+        const int n = ng;
+        const size_t nn = n2;
+        fwd2d(nlev, ld(fp), [=] __device__(int s, int ky, int kx, T v) { S[s * nn + kx + (size_t)n * ky] = v * F[kx + (size_t)n * ky]; });
+        inv2d(nlev, ld(S), stp(fp));
+    }
+
+    void divs(const T* a, const T* b, T* cs) {  // spectral/mod.rs:461-475
+        T* S = W("divs", 2 * n2);
+        const int n = ng;
+        const size_t nn = n2;
+        fwd2d(2, [=] __device__(int s, int x, int y) { return (s ? b : a)[y + (size_t)n * x]; }, stp(S));
+        each(st, n2, [=] __device__(size_t p) {
+            int kx = p % n, ky = p / n;
+            T xd = dmul<T>(kx, n) * S[mir(kx, n) + (size_t)n * ky];
+            T yd = dmul<T>(ky, n) * S[nn + kx + (size_t)n * mir(ky, n)];
+            cs[p] = yd + xd;
+        });
+    }
+
+    void jacob(const T* a, const T* b, T* cs) {  // spectral/mod.rs:423-457
+        T* S = W("jacS", 2 * n2);
+        T* D = W("jacD", 4 * n2);
+        const int n = ng;
+        const size_t nn = n2;
+        fwd2d(2, [=] __device__(int s, int x, int y) { return (s ? b : a)[y + (size_t)n * x]; }, stp(S));
+        inv2d(4, [=] __device__(int s, int ky, int kx) {
+            const T* g = S + (s >> 1) * nn;
+            return (s & 1) ? dmul<T>(ky, n) * g[kx + (size_t)n * mir(ky, n)] : dmul<T>(kx, n) * g[mir(kx, n) + (size_t)n * ky];
+        }, stp(D));
+        fwd2d(1, [=] __device__(int, int x, int y) {
+            size_t q = y + (size_t)n * x;
+            return D[q] * D[3 * nn + q] - D[nn + q] * D[2 * nn + q];
+        }, stp(cs));
+    }
+
+#include "sw_engine_ops.inc"
+};
+
+}  // namespace sw

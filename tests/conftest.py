@@ -28,3 +28,28 @@ def oracle():
     import _oracle
     _oracle.lib()
     return _oracle
+
+
+def load_sw():
+    """Import the package from shallow-water_amd/ (its directory name is not an identifier)."""
+    import importlib.util
+    if "shallow_water_amd" in sys.modules:
+        return sys.modules["shallow_water_amd"]
+    pkg = os.path.join(ROOT, "shallow-water_amd")
+    spec = importlib.util.spec_from_file_location("shallow_water_amd", os.path.join(pkg, "__init__.py"),
+                                                  submodule_search_locations=[pkg])
+    mod = importlib.util.module_from_spec(spec)
+    sys.modules["shallow_water_amd"] = mod
+    spec.loader.exec_module(mod)
+    return mod
+
+
+@pytest.fixture(scope="session")
+def sw():
+    return load_sw()
+
+
+def rel_l2(a, b):
+    a, b = np.asarray(a), np.asarray(b)
+    den = np.linalg.norm(b)
+    return np.linalg.norm(a - b) / (den if den > 0 else 1.0)

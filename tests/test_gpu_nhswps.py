@@ -1,0 +1,116 @@
+"""GPU parity of the nhswps kernels and the whole timestep (src/nhswps) against the reference's
+golden vectors and the oracle.  The step bar: relative L2 <= 1e-12 per field after one step."""
+import numpy as np
+import pytest
+from conftest import gold, rel_l2
+
+pytestmark = pytest.mark.gpu
+D = "nhswps/testdata/"
+ALL = ["ri", "r", "u", "v", "w", "zeta", "z", "zx", "zy", "ps", "pn", "dpn", "aa", "qs", "ds", "gs"]
+
+
+def pair(sw, oracle, ng, nz, prefix, names, **kw):
+    files = {n: gold(prefix + n + ".bin") for n in names}
+    return (sw.State(sw.Spectral(ng, nz), **files, **kw), oracle.State(oracle.Spectral(ng, nz), **files, **kw))
+
+
+@pytest.mark.parametrize("ng,nz", [(18, 2), (32, 4)])
+def test_vertical(sw, oracle, ng, nz):
+    p = f"{D}vertical/{ng}_{nz}_"
+    g, o = pair(sw, oracle, ng, nz, p, ["z", "zx", "zy", "r", "w", "aa", "u", "v", "ds"])
+    sw.vertical(g)
+    oracle.vertical(o)
+    for f in ("z", "zx", "zy", "w", "aa"):
+        np.testing.assert_allclose(g.get(f), gold(p + f + "2.bin"), rtol=0, atol=1e-10, err_msg=f)
+        assert rel_l2(g.get(f), getattr(o, f)) < 1e-13, f
+
+
+@pytest.mark.parametrize("ng,nz", [(18, 2), (32, 4)])
+def test_coeffs(sw, oracle, ng, nz):
+    p = f"{D}coeffs/{ng}_{nz}_"
+    g, o = pair(sw, oracle, ng, nz, p, ["ri", "zx", "zy"])
+    out = [g.spectral.field() for _ in range(4)]
+    ref = [o.spectral.field() for _ in range(4)]
+    sw.coeffs(g, *out)
+    oracle.coeffs(o, *ref)
+    for name, a, b in zip(("sigx", "sigy", "cpt1", "cpt2"), out, ref):
+        np.testing.assert_allclose(a, gold(p + name + "2.bin"), rtol=0, atol=1e-10, err_msg=name)
+        assert rel_l2(a, b) < 1e-13, name
+
+
+@pytest.mark.parametrize("ng,nz", [(18, 2), (32, 4)])
+def test_cpsource(sw, oracle, ng, nz):
+    p = f"{D}cpsource/{ng}_{nz}_"
+    g, o = pair(sw, oracle, ng, nz, p, ["ri", "u", "v", "w", "zeta", "z", "zx", "zy"])
+    a, b = g.spectral.field(), o.spectral.field()
+    sw.cpsource(g, a)
+    oracle.cpsource(o, b)
+    np.testing.assert_allclose(a, gold(p + "sp02.bin"), rtol=0, atol=1e-12)
+    assert rel_l2(a, b) < 1e-13
+
+
+@pytest.mark.parametrize("ng,nz", [(18, 2), (32, 4)])
+def test_source(sw, oracle, ng, nz):
+    p = f"{D}source/{ng}_{nz}_"
+    g, o = pair(sw, oracle, ng, nz, p, ["aa", "qs", "ds", "ps", "u", "v", "ri", "dpn", "z", "zx", "zy"])
+    out = [g.spectral.field() for _ in range(3)]
+    ref = [o.spectral.field() for _ in range(3)]
+    sw.source(g, *out)
+    oracle.source(o, *ref)
+    for name, a, b in zip(("sqs", "sds", "sgs"), out, ref):
+        np.testing.assert_allclose(a, gold(p + name + "2.bin"), rtol=0, atol=1e-12, err_msg=name)
+        assert rel_l2(a, b) < 1e-13, name
+
+
+@pytest.mark.parametrize("ng,nz", [(24, 4), (32, 4)])
+def test_psolve(sw, oracle, ng, nz):
+    p = f"{D}psolve/{ng}_{nz}_"
+    g, o = pair(sw, oracle, ng, nz, p, ALL)
+    it, errp = sw.psolve(g)
+    rc, it2, errp2 = oracle.psolve(o)
+    assert rc == 0 and it == it2, (it, it2)
+    for f in ("z", "zx", "zy", "w", "aa", "ri", "pn", "ps", "dpn"):
+        np.testing.assert_allclose(g.get(f), gold(p + f + "2.bin"), rtol=0, atol=1e-10, err_msg=f)
+        assert rel_l2(g.get(f), getattr(o, f)) < 1e-12, f
+
+
+def test_psolve_iteration_cap(sw):
+    p = f"{D}psolve/24_4_"
+    g = sw.State(sw.Spectral(24, 4), **{n: gold(p + n + ".bin") for n in ALL})
+    with pytest.raises(sw.SwError) as e:
+        sw.psolve(g, nitmax=1)
+    assert e.value.code == -2
+
+
+@pytest.mark.parametrize("ng,nz,t0", [(18, 6, 5.555_555_555_555_555E-2), (24, 4, 0.624_999_999_999_999_9)])
+def test_advance_golden(sw, oracle, ng, nz, t0):
+    # src/nhswps/advance.rs:296-600: 16 fields and t at abs 1e-10
+    p = f"{D}advance/{ng}_{nz}/"
+    g, o = pair(sw, oracle, ng, nz, p, ALL, t=t0, ggen=True)
+    diag, iters = sw.advance(g)
+    rc, diag2, iters2 = oracle.advance(o)
+    assert iters == iters2
+    assert abs(g.t - o.t) < 1e-15
+    np.testing.assert_allclose(diag, diag2, rtol=1e-12)
+    for f in ALL:
+        np.testing.assert_allclose(g.get(f), gold(p + f + "2.bin"), rtol=0, atol=1e-10, err_msg=f)
+        assert rel_l2(g.get(f), getattr(o, f)) < 1e-12, f
+
+
+def synthetic(sw, oracle, ng, nz, seed):
+    qs, ds = sw.synth_kminus3(ng, nz, seed=seed, k0=8.0, delta_rms=0.01 * 4 * np.pi)
+    zero = np.zeros_like(qs)
+    o = oracle.State(oracle.Spectral(ng, nz), qs=qs, ds=ds, gs=zero, ggen=True)
+    o.spectral.main_invert(o.qs, o.ds, o.gs, o.r, o.u, o.v, o.zeta)
+    g = sw.State(sw.Spectral(ng, nz), ggen=True, **{f: getattr(o, f) for f in ALL})
+    return g, o
+
+
+@pytest.mark.parametrize("ng,nz", [(64, 4), (96, 6)])
+def test_one_step_synthetic(sw, oracle, ng, nz):
+    g, o = synthetic(sw, oracle, ng, nz, seed=1)
+    diag, iters = sw.advance(g)
+    rc, diag2, iters2 = oracle.advance(o)
+    assert rc == 0 and iters == iters2
+    for f in ALL:
+        assert rel_l2(g.get(f), getattr(o, f)) < 1e-12, f
