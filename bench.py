@@ -1,0 +1,187 @@
+#!/usr/bin/env python3
+"""Throughput of the nhswps timestep (src/nhswps/advance.rs) on one MI355X.
+
+A step is one full `advance` (predictor + two correctors: 3x main_invert, 3x psolve, 3x source)
+with all fields resident in HBM, on a synthetic seeded k^-3 PV state.  The headline workload is
+ng=512, nz=16: the mixed-precision step (FP32 transforms, inversion and sources; FP64 pressure
+tridiagonal and residual) is timed beside the FP64 step, and its per-field relative L2 error
+against the FP64 step after one step is reported.  A point-step is one stored level point advanced
+one step: ng^2 * (nz+1) per step.
+
+Also reported: the roofline of the straight FFT pass kernel (HIP events on the library stream
+around its launches; algorithmic bytes = read + write of every transformed value) and a CPU
+baseline, the oracle (C++ restatement of the reference algorithm, OpenMP over levels) timed on
+this host on a bounded sample (one step).
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--ng 512] [--nz 16]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+METRIC = "point-steps/s @ng512 nz16: mixed vs FP64 GPU at rel-L2≤1e-5; % HBM roofline"
+HBM_PEAK_GBS = 8000.0
+FIELDS = ["u", "v", "w", "z", "zx", "zy", "r", "ri", "aa", "zeta", "pn", "dpn", "ps", "qs", "ds", "gs"]
+
+
+def load_sw():
+    import importlib.util
+    pkg = os.path.join(ROOT, "shallow-water_amd")
+    spec = importlib.util.spec_from_file_location("shallow_water_amd", os.path.join(pkg, "__init__.py"),
+                                                  submodule_search_locations=[pkg])
+    mod = importlib.util.module_from_spec(spec)
+    sys.modules["shallow_water_amd"] = mod
+    spec.loader.exec_module(mod)
+    return mod
+
+
+def initial_fields(sw, ng, nz, seed):
+    """Seeded k^-3 PV (k0 = 8, rms zeta = f/2) on identical levels, delta = gamma = 0, pn = 0, then
+    the loader's inversion (src/nhswps/mod.rs:238-277) done on the GPU in FP64."""
+    qs, ds = sw.synth_kminus3(ng, nz, seed=seed, k0=8.0, zeta_rms=0.5 * 4 * np.pi)
+    gs = np.zeros_like(qs)
+    spec = sw.Spectral(ng, nz, sw.FP64)
+    r, u, v, zeta = spec.field(), spec.field(), spec.field(), spec.field()
+    spec.main_invert(qs, ds, gs, r, u, v, zeta)
+    return dict(qs=qs, ds=ds, gs=gs, r=r, u=u, v=v, zeta=zeta)
+
+
+def new_state(sw, ng, nz, precision, init):
+    spec = sw.Spectral(ng, nz, precision)
+    st = sw.State(spec, t=0.0, ggen=True, **init)
+    return spec, st
+
+
+def time_steps(sw, spec, st, steps, warmup):
+    for _ in range(warmup):
+        st.set_time(st.t, True)
+        sw.advance(st)
+    sw.sync()
+    iters = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        st.set_time(st.t, True)
+        _, it = sw.advance(st)
+        iters.append(sum(it))
+    sw.sync()
+    dt = time.perf_counter() - t0
+    # the same workload again with events around the straight FFT pass launches
+    spec.profile(True)
+    for _ in range(min(steps, 2)):
+        st.set_time(st.t, True)
+        sw.advance(st)
+    ms, nbytes, launches = spec.profile_read()
+    spec.profile(False)
+    return dt / steps * 1e3, iters, ms, nbytes, launches
+
+
+def rel_l2(a, b):
+    n = np.linalg.norm(b)
+    return float(np.linalg.norm(a - b) / (n if n > 0 else 1.0))
+
+
+def one_step_errors(sw, ng, nz, init):
+    out = {}
+    for p in (sw.FP64, sw.MIXED):
+        spec, st = new_state(sw, ng, nz, p, init)
+        sw.advance(st)
+        out[p] = {f: st.get(f) for f in ("qs", "ds", "gs", "z", "pn")}
+    n2 = ng * ng
+    err = {}
+    for name, f in (("q", "qs"), ("delta", "ds"), ("gamma", "gs"), ("p", "pn")):
+        err[name] = rel_l2(out[sw.MIXED][f], out[sw.FP64][f])
+    err["h"] = rel_l2(out[sw.MIXED]["z"][nz * n2:], out[sw.FP64]["z"][nz * n2:])
+    return err
+
+
+def cpu_baseline(sw, ng, nz, init):
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import _oracle
+    cores = min(os.cpu_count() or 1, 16)
+    cores = _oracle.set_threads(cores)
+    o = _oracle.State(_oracle.Spectral(ng, nz), ggen=True, **init)
+    t0 = time.perf_counter()
+    rc, _, _ = _oracle.advance(o)
+    dt = time.perf_counter() - t0
+    if rc != 0:
+        return None
+    return {"value": ng * ng * (nz + 1) / dt, "unit": "point-steps/s", "cores": cores, "kind": "port",
+            "sample": f"1 advance step at ng={ng}, nz={nz} (FP64, {dt:.1f} s), same seeded state"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--ng", type=int, default=512)
+    ap.add_argument("--nz", type=int, default=16)
+    ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path (members or levels would "
+              "shard across GPUs as described in DESIGN.md); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+    sw = load_sw()
+    ng, nz = args.ng, args.nz
+    points = ng * ng * (nz + 1)
+    init = initial_fields(sw, ng, nz, args.seed)
+
+    res = {}
+    for p, name in ((sw.FP64, "fp64"), (sw.MIXED, "mixed")):
+        spec, st = new_state(sw, ng, nz, p, init)
+        ms, iters, pms, pbytes, plaunch = time_steps(sw, spec, st, args.steps, args.warmup)
+        res[name] = dict(ms=ms, iters=iters, fft_ms=pms, fft_bytes=pbytes, fft_launches=plaunch,
+                         value=points / (ms * 1e-3))
+        print(f"[bench] {name}: {ms:.3f} ms/step, {points / (ms * 1e-3):.4g} point-steps/s, psolve iterations "
+              f"per step {iters}", file=sys.stderr, flush=True)
+        del st, spec
+    err = one_step_errors(sw, ng, nz, init)
+    print(f"[bench] mixed vs fp64 after 1 step: {err}", file=sys.stderr, flush=True)
+
+    m = res["mixed"]
+    achieved = m["fft_bytes"] / (m["fft_ms"] * 1e-3) / 1e9 if m["fft_ms"] > 0 else 0.0
+    f64 = res["fp64"]
+    achieved64 = f64["fft_bytes"] / (f64["fft_ms"] * 1e-3) / 1e9 if f64["fft_ms"] > 0 else 0.0
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = cpu_baseline(sw, ng, nz, init)
+    line = {
+        "metric": METRIC,
+        "value": m["value"],
+        "unit": "point-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": m["ms"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32+f64",
+        "data": "synthetic",
+        "config": {"workload": f"nhswps advance (3x main_invert, psolve, source), ng={ng}, nz={nz}, seeded k^-3 PV "
+                               f"(k0=8, rms zeta=f/2), delta=gamma=0, mixed precision", "ng": ng, "nz": nz,
+                   "precision": "mixed: f32 FFT/inversion/sources, f64 psolve tridiagonal+residual"},
+        "fp64": {"value": f64["value"], "ms_per_step": f64["ms"], "dtype": "f64",
+                 "psolve_iters_per_step": f64["iters"],
+                 "fft_pass_gbs": achieved64},
+        "mixed_over_fp64": m["value"] / f64["value"],
+        "psolve_iters_per_step": m["iters"],
+        "rel_l2_mixed_vs_fp64_after_1_step": err,
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "fft_fwd_pass/fft_inv_pass straight pass (mixed run)",
+                     "launches": m["fft_launches"], "avg_launch_us": m["fft_ms"] * 1e3 / max(m["fft_launches"], 1)},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -93,6 +93,14 @@ int sw_source(sw_state* st, double* sqs, double* sds, double* sgs);   /* source.
 int sw_diagnose(sw_state* st, double diag[5]);                        /* diagnose mod.rs:666 */
 int sw_advance(sw_state* st, double diag[5], int iters[3]);           /* advance.rs:13  */
 
+/* ---- instrumentation and controls (no reference counterpart) ---- */
+/* Time the straight FFT pass launches of this handle with HIP events on the library stream. */
+int sw_profile(sw_spectral* s, int enable);
+int sw_profile_read(sw_spectral* s, double* ms, double* bytes, long* launches);
+/* psolve stopping tolerance; default 1e-9 (psolve.rs:17) in FP64, 1e-6 in the mixed mode, where the
+ * FP32 transforms put a floor under the residual. */
+int sw_set_tolerance(sw_spectral* s, double toler);
+
 /* ---- synthetic initial state (host, no reference counterpart: the bench/test generator) ---- */
 int sw_synth_kminus3(size_t ng, size_t nz, unsigned long long seed, double k0, double zeta_rms,
                      double delta_rms, double* qq, double* dd);

@@ -150,6 +150,19 @@ class Spectral:
         _lib.sw_spectral_get(self._h, name.encode(), _d(out))
         return out
 
+    def profile(self, enable=True):
+        """Start (or stop) timing this handle's straight FFT pass launches with HIP events."""
+        _check(_lib.sw_profile(self._h, C.c_int(1 if enable else 0)))
+
+    def profile_read(self):
+        """(total ms, total algorithmic bytes, launches) of the timed FFT pass launches."""
+        ms, by, n = C.c_double(), C.c_double(), C.c_long()
+        _check(_lib.sw_profile_read(self._h, C.byref(ms), C.byref(by), C.byref(n)))
+        return ms.value, by.value, n.value
+
+    def set_tolerance(self, toler):
+        _check(_lib.sw_set_tolerance(self._h, C.c_double(toler)))
+
     def main_invert(self, qs, ds, gs, r, u, v, zeta):
         _check(_lib.sw_main_invert(self._h, _d(qs), _d(ds), _d(gs), _d(r), _d(u), _d(v), _d(zeta)))
 

@@ -73,7 +73,7 @@ struct Arg {
         if (!dev) {
             stage.alloc(n);
             dd = stage.p;
-            if (in) SW_HIP(hipMemcpyAsync(dd, user, n * sizeof(double), hipMemcpyHostToDevice, st));
+            if (in) copy_h2d(dd, user, n * sizeof(double), st);
         }
         if constexpr (sizeof(T) == sizeof(double)) {
             d = reinterpret_cast<T*>(dd);
@@ -88,8 +88,8 @@ struct Arg {
         double* dd = dev ? user : stage.p;
         if constexpr (sizeof(T) != sizeof(double))
             hipLaunchKernelGGL(k_cvt_f2d, dim3((n + 255) / 256), dim3(256), 0, st, conv.p, dd, n);
-        if (!dev) SW_HIP(hipMemcpyAsync(user, dd, n * sizeof(double), hipMemcpyDeviceToHost, st));
         SW_HIP(hipStreamSynchronize(st));
+        if (!dev) copy_d2h(user, dd, n * sizeof(double), st);
     }
 };
 
@@ -575,6 +575,41 @@ int sw_advance(sw_state* st, double diag[5], int iters[3]) {
         if (diag) std::memcpy(diag, dg, sizeof(dg));
         if (iters) std::memcpy(iters, it, sizeof(it));
         return rc;
+    });
+}
+
+int sw_profile(sw_spectral* s, int enable) {
+    return guard([&] {
+        return with_engine(s, [&](auto& e) {
+            e.prof_collect();
+            e.prof = enable != 0;
+            e.prof_ms = e.prof_bytes = 0.0;
+            e.prof_launches = 0;
+            return 0;
+        });
+    });
+}
+
+int sw_profile_read(sw_spectral* s, double* ms, double* bytes, long* launches) {
+    return guard([&] {
+        SW_CHECK(ms && bytes && launches, SW_ERR_ARG, "null argument");
+        return with_engine(s, [&](auto& e) {
+            e.prof_collect();
+            *ms = e.prof_ms;
+            *bytes = e.prof_bytes;
+            *launches = e.prof_launches;
+            return 0;
+        });
+    });
+}
+
+int sw_set_tolerance(sw_spectral* s, double toler) {
+    return guard([&] {
+        SW_CHECK(toler > 0.0, SW_ERR_ARG, "tolerance must be positive");
+        return with_engine(s, [&](auto& e) {
+            e.toler = toler;
+            return 0;
+        });
     });
 }
 

@@ -2,7 +2,10 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdio>
+#include <cstring>
+#include <mutex>
 #include <memory>
 #include <stdexcept>
 #include <string>
@@ -42,8 +45,13 @@ struct DevBuf {
         if (count) SW_HIP(hipMalloc(&p, count * sizeof(T)));
         n = count;
     }
+    // Work queued on the library's (non-blocking) stream may still read or write the old block:
+    // wait for the device before returning it to the allocator.
     void release() {
-        if (p) (void)hipFree(p);
+        if (p) {
+            (void)hipDeviceSynchronize();
+            (void)hipFree(p);
+        }
         p = nullptr;
         n = 0;
     }
@@ -73,6 +81,44 @@ inline void each(hipStream_t st, size_t n, F f) {
     unsigned blocks = (unsigned)((n + 255) / 256);
     hipLaunchKernelGGL(k_each<F>, dim3(blocks), dim3(256), 0, st, n, f);
     SW_HIP(hipGetLastError());
+}
+
+// Host <-> device copies through a page-locked block owned by the library (hipHostMalloc): the
+// runtime never has to lock the caller's pageable memory for a DMA.  Each chunk completes before
+// the block is reused.
+struct Pinned {
+    static std::mutex& mu() {
+        static std::mutex m;
+        return m;
+    }
+    static void* block() {
+        static void* p = nullptr;
+        if (!p) SW_HIP(hipHostMalloc(&p, chunk(), hipHostMallocDefault));
+        return p;
+    }
+    static size_t chunk() { return (size_t)8 << 20; }
+};
+
+inline void copy_h2d(void* dst, const void* src, size_t bytes, hipStream_t st) {
+    std::lock_guard<std::mutex> lk(Pinned::mu());
+    void* pin = Pinned::block();
+    for (size_t off = 0; off < bytes; off += Pinned::chunk()) {
+        size_t b = std::min(Pinned::chunk(), bytes - off);
+        std::memcpy(pin, (const char*)src + off, b);
+        SW_HIP(hipMemcpyAsync((char*)dst + off, pin, b, hipMemcpyHostToDevice, st));
+        SW_HIP(hipStreamSynchronize(st));
+    }
+}
+
+inline void copy_d2h(void* dst, const void* src, size_t bytes, hipStream_t st) {
+    std::lock_guard<std::mutex> lk(Pinned::mu());
+    void* pin = Pinned::block();
+    for (size_t off = 0; off < bytes; off += Pinned::chunk()) {
+        size_t b = std::min(Pinned::chunk(), bytes - off);
+        SW_HIP(hipMemcpyAsync(pin, (const char*)src + off, b, hipMemcpyDeviceToHost, st));
+        SW_HIP(hipStreamSynchronize(st));
+        std::memcpy((char*)dst + off, pin, b);
+    }
 }
 
 // True when p is device (or managed) memory the GPU can address directly.

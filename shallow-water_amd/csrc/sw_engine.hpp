@@ -135,6 +135,50 @@ struct Engine {
     DevBuf<T> tmp;  // intermediate of the two FFT passes
     DevBuf<double> part, scal, dwork;
     std::map<std::string, DevBuf<T>> work;
+    // live timing of the straight (second) FFT pass, the kernel the roofline is quoted for:
+    // events on this engine's stream around each launch, algorithmic bytes = read + write of
+    // nslab * ng^2 values
+    bool prof = false;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> ev;
+    std::vector<double> ev_bytes;
+    size_t ev_used = 0;
+    double prof_ms = 0.0, prof_bytes = 0.0;
+    long prof_launches = 0;
+    void prof_start(double bytes) {
+        if (!prof) return;
+        if (ev_used == ev.size()) {
+            hipEvent_t a, b;
+            SW_HIP(hipEventCreate(&a));
+            SW_HIP(hipEventCreate(&b));
+            ev.push_back({a, b});
+            ev_bytes.push_back(0.0);
+        }
+        ev_bytes[ev_used] = bytes;
+        SW_HIP(hipEventRecord(ev[ev_used].first, st));
+    }
+    void prof_stop() {
+        if (!prof) return;
+        SW_HIP(hipEventRecord(ev[ev_used].second, st));
+        ev_used++;
+    }
+    void prof_collect() {
+        if (!ev_used) return;
+        SW_HIP(hipStreamSynchronize(st));
+        for (size_t i = 0; i < ev_used; i++) {
+            float ms = 0.f;
+            SW_HIP(hipEventElapsedTime(&ms, ev[i].first, ev[i].second));
+            prof_ms += ms;
+            prof_bytes += ev_bytes[i];
+            prof_launches++;
+        }
+        ev_used = 0;
+    }
+    ~Engine() {
+        for (auto& e : ev) {
+            (void)hipEventDestroy(e.first);
+            (void)hipEventDestroy(e.second);
+        }
+    }
     // psolve controls
     double toler = 1.0e-9;
     int last_iters = 0;
@@ -152,11 +196,11 @@ struct Engine {
         auto up = [&](DevBuf<T>& d, const std::vector<double>& h) {
             std::vector<T> c(h.begin(), h.end());
             d.alloc(c.size());
-            SW_HIP(hipMemcpy(d.p, c.data(), c.size() * sizeof(T), hipMemcpyHostToDevice));
+            copy_h2d(d.p, c.data(), c.size() * sizeof(T), st);
         };
         auto upd = [&](DevBuf<double>& d, const std::vector<double>& h) {
             d.alloc(h.size());
-            SW_HIP(hipMemcpy(d.p, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice));
+            copy_h2d(d.p, h.data(), h.size() * sizeof(double), st);
         };
         up(hlap, host.hlap); up(glap, host.glap); up(rlap, host.rlap); up(c2g2, host.c2g2);
         up(simp, host.simp); up(rope, host.rope); up(fope, host.fope); up(filt, host.filt);
@@ -171,6 +215,7 @@ struct Engine {
             upd(vc, h);
         }
         part.alloc((size_t)RED_BLOCKS * 4 * (MAX_NZ + 1));
+        tmp.alloc((size_t)9 * L * n2);  // the largest batch (source: 9 levels' worth of slabs)
         scal.alloc(16);
         if (mixed) toler = 1.0e-6;
     }
@@ -206,7 +251,7 @@ struct Engine {
             h[k] = C2<T>{T(std::cos(a)), T(std::sin(a))};
         }
         twb.alloc(n);
-        SW_HIP(hipMemcpy(twb.p, h.data(), n * sizeof(C2<T>), hipMemcpyHostToDevice));
+        copy_h2d(twb.p, h.data(), n * sizeof(C2<T>), nullptr);
         pl.tw = twb.p;
     }
 
@@ -235,8 +280,10 @@ struct Engine {
         hipLaunchKernelGGL((fft_fwd_pass<T, true, Pro, NoEpi>), grid, block, lds_bytes, st, plan, nslab, pro, NoEpi{}, t);
         SW_HIP(hipGetLastError());
         LoadPlain<T> lp{t, n2, ng};
+        prof_start(2.0 * sizeof(T) * (double)n2 * nslab);
         hipLaunchKernelGGL((fft_fwd_pass<T, false, LoadPlain<T>, Epi>), grid, block, lds_bytes, st, plan, nslab, lp, epi,
                            (T*)nullptr);
+        prof_stop();
         SW_HIP(hipGetLastError());
     }
 
@@ -250,8 +297,10 @@ struct Engine {
         hipLaunchKernelGGL((fft_inv_pass<T, true, Pro, NoEpi>), grid, block, lds_bytes, st, plan, nslab, pro, NoEpi{}, t);
         SW_HIP(hipGetLastError());
         LoadPlain<T> lp{t, n2, ng};
+        prof_start(2.0 * sizeof(T) * (double)n2 * nslab);
         hipLaunchKernelGGL((fft_inv_pass<T, false, LoadPlain<T>, Epi>), grid, block, lds_bytes, st, plan, nslab, lp, epi,
                            (T*)nullptr);
+        prof_stop();
         SW_HIP(hipGetLastError());
     }
 
@@ -265,8 +314,7 @@ struct Engine {
         hipLaunchKernelGGL((k_reduce<K, MAXMASK, F>), dim3(blocks), dim3(256), 0, st, n, f, part.p);
         SW_HIP(hipGetLastError());
         std::vector<double> h((size_t)blocks * K);
-        SW_HIP(hipMemcpyAsync(h.data(), part.p, h.size() * sizeof(double), hipMemcpyDeviceToHost, st));
-        SW_HIP(hipStreamSynchronize(st));
+        copy_d2h(h.data(), part.p, h.size() * sizeof(double), st);
         for (int c = 0; c < K; c++) {
             bool mx = (MAXMASK >> c) & 1;
             double acc = mx ? -INFINITY : 0.0, cmp = 0.0;

@@ -10,6 +10,15 @@ sys.path.insert(0, os.path.join(ROOT, "tests"))
 sys.path.insert(0, ROOT)
 
 
+@pytest.fixture(autouse=True)
+def _free_gpu_objects():
+    """Release the GPU handles of a test (and of earlier failed tests) in the main thread at a known
+    point, instead of whenever the garbage collector runs."""
+    yield
+    import gc
+    gc.collect()
+
+
 def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an MI355X (gfx950) GPU and the HIP library")
 
