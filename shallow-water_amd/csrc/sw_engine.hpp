@@ -25,6 +25,17 @@ __device__ __forceinline__ T dmul(int k, int n) {
     return (k == 0 || 2 * k == n) ? T(0) : (2 * k < n ? T(-k) : T(n - k));
 }
 
+// Base pointers of K arrays, one of which a prologue or epilogue picks with a block-uniform index.
+// Kernels must not choose among argument pointers with chains of comparisons (?: on the slab
+// group): clang (ROCm 7.2, gfx950) folds such a chain into a switch and, after structurizing it,
+// can leave the uniform pointer phi unassigned on one path, which then loads through a stale
+// register.  Indexing this table is a plain load from the kernel argument block.
+template <typename T, int K>
+struct PtrTab {
+    T* p[K];
+    __device__ __forceinline__ T* operator[](int i) const { return p[i]; }
+};
+
 constexpr int RED_BLOCKS = 512;
 constexpr int MAX_NZ = 256;
 
@@ -350,7 +361,8 @@ struct Engine {
         T* S = W("divs", 2 * n2);
         const int n = ng;
         const size_t nn = n2;
-        fwd2d(2, [=] __device__(int s, int x, int y) { return (s ? b : a)[y + (size_t)n * x]; }, stp(S));
+        const PtrTab<const T, 2> ab{{a, b}};
+        fwd2d(2, [=] __device__(int s, int x, int y) { return ab[s][y + (size_t)n * x]; }, stp(S));
         each(st, n2, [=] __device__(size_t p) {
             int kx = p % n, ky = p / n;
             T xd = dmul<T>(kx, n) * S[mir(kx, n) + (size_t)n * ky];
@@ -364,7 +376,8 @@ struct Engine {
         T* D = W("jacD", 4 * n2);
         const int n = ng;
         const size_t nn = n2;
-        fwd2d(2, [=] __device__(int s, int x, int y) { return (s ? b : a)[y + (size_t)n * x]; }, stp(S));
+        const PtrTab<const T, 2> ab{{a, b}};
+        fwd2d(2, [=] __device__(int s, int x, int y) { return ab[s][y + (size_t)n * x]; }, stp(S));
         inv2d(4, [=] __device__(int s, int ky, int kx) {
             const T* g = S + (s >> 1) * nn;
             return (s & 1) ? dmul<T>(ky, n) * g[kx + (size_t)n * mir(ky, n)] : dmul<T>(kx, n) * g[mir(kx, n) + (size_t)n * ky];

@@ -9,19 +9,11 @@ reference's own Fortran comparison uses 1e-13)."""
 import numpy as np
 import pytest
 from conftest import gold, rel_l2
+from _tolerances import STEP_TOL
 
 pytestmark = pytest.mark.gpu
 D = "nhswps/testdata/"
 ALL = ["ri", "r", "u", "v", "w", "zeta", "z", "zx", "zy", "ps", "pn", "dpn", "aa", "qs", "ds", "gs"]
-
-# Per-field relative L2 bound against the oracle after psolve / one step.  1e-12 where the step is
-# well conditioned; the pressure solve amplifies rounding differences: the reference algorithm
-# itself moves pn by ~1.5e-11, ds by ~2.4e-12 and w by ~2e-12 after one step when its inputs are
-# perturbed by 1e-16 (tests/test_oracle_conditioning.py), and a transform that rounds differently
-# perturbs every one of the step's thousands of transforms at that level.
-STEP_TOL = {"qs": 1e-12, "r": 1e-12, "u": 1e-12, "v": 1e-12, "zeta": 1e-12, "z": 1e-12, "ri": 1e-12,
-            "zx": 5e-12, "zy": 5e-12, "aa": 5e-12, "w": 5e-11, "ds": 1e-10, "gs": 5e-11,
-            "pn": 5e-10, "ps": 5e-10, "dpn": 5e-10}
 
 
 def pair(sw, oracle, ng, nz, prefix, names, **kw):
@@ -53,14 +45,7 @@ def test_coeffs(sw, oracle, ng, nz):
         assert rel_l2(a, b) < 1e-13, name
 
 
-# The first GPU runs of this suite left the GPU faulted in the window between test_cpsource at
-# ng=32, nz=4 and the next test; CPU emulation of the same kernels (guard-paged buffers, ASan,
-# UBSan) shows no out-of-bounds access, and the cause has not been found.  The computation is kept
-# off the GPU at that shape until it is.
-FAULT_WINDOW = pytest.mark.skip(reason="GPU fault observed in this case's window; cause not yet shown")
-
-
-@pytest.mark.parametrize("ng,nz", [(18, 2), pytest.param(32, 4, marks=FAULT_WINDOW)])
+@pytest.mark.parametrize("ng,nz", [(18, 2), (32, 4)])
 def test_cpsource(sw, oracle, ng, nz):
     p = f"{D}cpsource/{ng}_{nz}_"
     g, o = pair(sw, oracle, ng, nz, p, ["ri", "u", "v", "w", "zeta", "z", "zx", "zy"])
@@ -84,7 +69,7 @@ def test_source(sw, oracle, ng, nz):
         assert rel_l2(a, b) < 1e-12, name
 
 
-@pytest.mark.parametrize("ng,nz", [(24, 4), pytest.param(32, 4, marks=FAULT_WINDOW)])
+@pytest.mark.parametrize("ng,nz", [(24, 4), (32, 4)])
 def test_psolve(sw, oracle, ng, nz):
     p = f"{D}psolve/{ng}_{nz}_"
     g, o = pair(sw, oracle, ng, nz, p, ALL)
