@@ -123,7 +123,21 @@ def main():
     ap.add_argument("--nz", type=int, default=16)
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run a plain device copy of MIB MiB (twice) and exit: the reference reading "
+                         "for the FETCH_SIZE/WRITE_SIZE counters")
     args = ap.parse_args()
+    if args.copy_calibration:
+        import torch
+        n = args.copy_calibration * (1 << 20) // 8
+        a = torch.ones(n, dtype=torch.float64, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(2):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(f"[bench] copy calibration: 2 copies of {args.copy_calibration} MiB (read {args.copy_calibration} MiB "
+              f"+ write {args.copy_calibration} MiB each)", file=sys.stderr, flush=True)
+        return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path (members or levels would "
               "shard across GPUs as described in DESIGN.md); run with --gpus 1", file=sys.stderr)
