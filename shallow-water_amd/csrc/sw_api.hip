@@ -578,11 +578,12 @@ int sw_advance(sw_state* st, double diag[5], int iters[3]) {
     });
 }
 
-int sw_profile(sw_spectral* s, int enable) {
+int sw_profile(sw_spectral* s, int mode) {
     return guard([&] {
+        SW_CHECK(mode >= SW_PROF_OFF && mode <= SW_PROF_PSOLVE_DERIV, SW_ERR_ARG, "unknown profiling mode");
         return with_engine(s, [&](auto& e) {
             e.prof_collect();
-            e.prof = enable != 0;
+            e.prof = mode;
             e.prof_ms = e.prof_bytes = 0.0;
             e.prof_launches = 0;
             return 0;

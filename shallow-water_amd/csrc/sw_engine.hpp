@@ -146,17 +146,15 @@ struct Engine {
     DevBuf<T> tmp;  // intermediate of the two FFT passes
     DevBuf<double> part, scal, dwork;
     std::map<std::string, DevBuf<T>> work;
-    // live timing of the straight (second) FFT pass, the kernel the roofline is quoted for:
-    // events on this engine's stream around each launch, algorithmic bytes = read + write of
-    // nslab * ng^2 values
-    bool prof = false;
+    // live timing of one kind of FFT pass (sw_profile modes): events on this engine's stream around
+    // each launch of it, algorithmic bytes = read + write of nslab * ng^2 values
+    int prof = SW_PROF_OFF;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> ev;
     std::vector<double> ev_bytes;
     size_t ev_used = 0;
     double prof_ms = 0.0, prof_bytes = 0.0;
     long prof_launches = 0;
     void prof_start(double bytes) {
-        if (!prof) return;
         if (ev_used == ev.size()) {
             hipEvent_t a, b;
             SW_HIP(hipEventCreate(&a));
@@ -168,7 +166,6 @@ struct Engine {
         SW_HIP(hipEventRecord(ev[ev_used].first, st));
     }
     void prof_stop() {
-        if (!prof) return;
         SW_HIP(hipEventRecord(ev[ev_used].second, st));
         ev_used++;
     }
@@ -277,6 +274,9 @@ struct Engine {
         return tmp.p;
     }
 
+    // call sites a profiling mode can single out
+    enum { SITE_OTHER = 0, SITE_PSOLVE_DERIV = 1 };
+
     struct NoEpi {
         __device__ void operator()(int, int, int, T) const {}
     };
@@ -284,34 +284,42 @@ struct Engine {
     // Forward 2-D transform of nslab slabs: pro(s, x, y) gives the physical value at (y, x);
     // epi(s, ky, kx, v) receives the spectral coefficient at (kx, ky).
     template <class Pro, class Epi>
-    void fwd2d(int nslab, Pro pro, Epi epi) {
+    void fwd2d(int nslab, Pro pro, Epi epi, int site = SITE_OTHER) {
         if (nslab <= 0) return;
         T* t = tmp_for(nslab);
         dim3 grid(nslab * (ng / plan.B)), block(plan.nt);
+        const double bytes = 2.0 * sizeof(T) * (double)n2 * nslab;
+        const bool time1 = prof == SW_PROF_PSOLVE_DERIV && site == SITE_PSOLVE_DERIV, time2 = prof == SW_PROF_STRAIGHT;
+        if (time1) prof_start(bytes);
         hipLaunchKernelGGL((fft_fwd_pass<T, true, Pro, NoEpi>), grid, block, lds_bytes, st, plan, nslab, pro, NoEpi{}, t);
+        if (time1) prof_stop();
         SW_HIP(hipGetLastError());
         LoadPlain<T> lp{t, n2, ng};
-        prof_start(2.0 * sizeof(T) * (double)n2 * nslab);
+        if (time2) prof_start(bytes);
         hipLaunchKernelGGL((fft_fwd_pass<T, false, LoadPlain<T>, Epi>), grid, block, lds_bytes, st, plan, nslab, lp, epi,
                            (T*)nullptr);
-        prof_stop();
+        if (time2) prof_stop();
         SW_HIP(hipGetLastError());
     }
 
     // Inverse 2-D transform: pro(s, ky, kx) gives the spectral coefficient at (kx, ky);
     // epi(s, x, y, v) receives the physical value at (y, x).
     template <class Pro, class Epi>
-    void inv2d(int nslab, Pro pro, Epi epi) {
+    void inv2d(int nslab, Pro pro, Epi epi, int site = SITE_OTHER) {
         if (nslab <= 0) return;
         T* t = tmp_for(nslab);
         dim3 grid(nslab * (ng / plan.B)), block(plan.nt);
+        const double bytes = 2.0 * sizeof(T) * (double)n2 * nslab;
+        const bool time1 = prof == SW_PROF_PSOLVE_DERIV && site == SITE_PSOLVE_DERIV, time2 = prof == SW_PROF_STRAIGHT;
+        if (time1) prof_start(bytes);
         hipLaunchKernelGGL((fft_inv_pass<T, true, Pro, NoEpi>), grid, block, lds_bytes, st, plan, nslab, pro, NoEpi{}, t);
+        if (time1) prof_stop();
         SW_HIP(hipGetLastError());
         LoadPlain<T> lp{t, n2, ng};
-        prof_start(2.0 * sizeof(T) * (double)n2 * nslab);
+        if (time2) prof_start(bytes);
         hipLaunchKernelGGL((fft_inv_pass<T, false, LoadPlain<T>, Epi>), grid, block, lds_bytes, st, plan, nslab, lp, epi,
                            (T*)nullptr);
-        prof_stop();
+        if (time2) prof_stop();
         SW_HIP(hipGetLastError());
     }
 
