@@ -13,6 +13,7 @@
 
 #include "sw_common.hpp"
 #include "sw_fft.hpp"
+#include "sw_fft_wave.hpp"
 #include "sw_host.hpp"
 
 namespace sw {
@@ -286,6 +287,10 @@ struct Engine {
     template <class Pro, class Epi>
     void fwd2d(int nslab, Pro pro, Epi epi, int site = SITE_OTHER) {
         if (nslab <= 0) return;
+        if (ng == 512) {
+            fwd2d_wave<512>(nslab, pro, epi, site);
+            return;
+        }
         T* t = tmp_for(nslab);
         dim3 grid(nslab * (ng / plan.B)), block(plan.nt);
         const double bytes = 2.0 * sizeof(T) * (double)n2 * nslab;
@@ -307,6 +312,10 @@ struct Engine {
     template <class Pro, class Epi>
     void inv2d(int nslab, Pro pro, Epi epi, int site = SITE_OTHER) {
         if (nslab <= 0) return;
+        if (ng == 512) {
+            inv2d_wave<512>(nslab, pro, epi, site);
+            return;
+        }
         T* t = tmp_for(nslab);
         dim3 grid(nslab * (ng / plan.B)), block(plan.nt);
         const double bytes = 2.0 * sizeof(T) * (double)n2 * nslab;
@@ -319,6 +328,47 @@ struct Engine {
         if (time2) prof_start(bytes);
         hipLaunchKernelGGL((fft_inv_pass<T, false, LoadPlain<T>, Epi>), grid, block, lds_bytes, st, plan, nslab, lp, epi,
                            (T*)nullptr);
+        if (time2) prof_stop();
+        SW_HIP(hipGetLastError());
+    }
+
+    // The same two passes with the wave-per-sequence kernels (sw_fft_wave.hpp) for n = NN.  The
+    // transposing pass groups NWT waves so that its store writes 64-byte runs; the straight pass runs
+    // one wave per block.
+    static constexpr int NWT = 32 / (int)sizeof(T);
+    template <int NN, class Pro, class Epi>
+    void fwd2d_wave(int nslab, Pro pro, Epi epi, int site) {
+        T* t = tmp_for(nslab);
+        const double bytes = 2.0 * sizeof(T) * (double)n2 * nslab;
+        const bool time1 = prof == SW_PROF_PSOLVE_DERIV && site == SITE_PSOLVE_DERIV, time2 = prof == SW_PROF_STRAIGHT;
+        const size_t lds1 = wave_lds_bytes<NN, NWT, T>(), lds2 = wave_lds_bytes<NN, 1, T>();
+        if (time1) prof_start(bytes);
+        hipLaunchKernelGGL((fftw_fwd_pass<T, NN, NWT, true, Pro, NoEpi>), dim3(nslab * (NN / (2 * NWT))), dim3(64 * NWT),
+                           lds1, st, plan.tw, nslab, pro, NoEpi{}, t);
+        if (time1) prof_stop();
+        SW_HIP(hipGetLastError());
+        LoadPlain<T> lp{t, n2, ng};
+        if (time2) prof_start(bytes);
+        hipLaunchKernelGGL((fftw_fwd_pass<T, NN, 1, false, LoadPlain<T>, Epi>), dim3(nslab * (NN / 2)), dim3(64), lds2, st,
+                           plan.tw, nslab, lp, epi, (T*)nullptr);
+        if (time2) prof_stop();
+        SW_HIP(hipGetLastError());
+    }
+    template <int NN, class Pro, class Epi>
+    void inv2d_wave(int nslab, Pro pro, Epi epi, int site) {
+        T* t = tmp_for(nslab);
+        const double bytes = 2.0 * sizeof(T) * (double)n2 * nslab;
+        const bool time1 = prof == SW_PROF_PSOLVE_DERIV && site == SITE_PSOLVE_DERIV, time2 = prof == SW_PROF_STRAIGHT;
+        const size_t lds1 = wave_lds_bytes<NN, NWT, T>(), lds2 = wave_lds_bytes<NN, 1, T>();
+        if (time1) prof_start(bytes);
+        hipLaunchKernelGGL((fftw_inv_pass<T, NN, NWT, true, Pro, NoEpi>), dim3(nslab * (NN / (2 * NWT))), dim3(64 * NWT),
+                           lds1, st, plan.tw, nslab, pro, NoEpi{}, t);
+        if (time1) prof_stop();
+        SW_HIP(hipGetLastError());
+        LoadPlain<T> lp{t, n2, ng};
+        if (time2) prof_start(bytes);
+        hipLaunchKernelGGL((fftw_inv_pass<T, NN, 1, false, LoadPlain<T>, Epi>), dim3(nslab * (NN / 2)), dim3(64), lds2, st,
+                           plan.tw, nslab, lp, epi, (T*)nullptr);
         if (time2) prof_stop();
         SW_HIP(hipGetLastError());
     }
