@@ -190,8 +190,8 @@ def main():
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(sw, ng, nz, init)
-    kname = {"fp64": "fft_inv_pass<double, true, psolve theta-derivative prologue>",
-             "mixed": "fft_inv_pass<float, true, psolve theta-derivative prologue>"}[head]
+    kname = {"fp64": "fftw_inv_pass<double, 512, 4, true, psolve theta-derivative prologue>",
+             "mixed": "fftw_inv_pass<float, 512, 8, true, psolve theta-derivative prologue>"}[head]
     line = {
         "metric": METRIC,
         "value": h["value"],
