@@ -60,7 +60,7 @@ def new_state(sw, ng, nz, precision, init):
     return spec, st
 
 
-def time_steps(sw, spec, st, steps, warmup, prof_mode):
+def time_steps(sw, spec, st, steps, warmup, prof_modes):
     for _ in range(warmup):
         st.set_time(st.t, True)
         sw.advance(st)
@@ -73,21 +73,26 @@ def time_steps(sw, spec, st, steps, warmup, prof_mode):
         iters.append(sum(it))
     sw.sync()
     dt = time.perf_counter() - t0
-    # the same workload again with events around the launches of the profiled kernel
-    spec.profile(prof_mode)
-    for _ in range(min(steps, 2)):
-        st.set_time(st.t, True)
-        sw.advance(st)
-    ms, nbytes, launches = spec.profile_read()
-    spec.profile(sw.PROF_OFF)
-    return dt / steps * 1e3, iters, ms, nbytes, launches
+    # the same workload again, per profiled kernel, with HIP events on the library stream around each
+    # launch of that kernel
+    prof = {}
+    for name, mode in prof_modes.items():
+        spec.profile(mode)
+        nstep = min(steps, 2)
+        for _ in range(nstep):
+            st.set_time(st.t, True)
+            sw.advance(st)
+        ms, nbytes, launches = spec.profile_read()
+        spec.profile(sw.PROF_OFF)
+        prof[name] = dict(ms_per_step=ms / nstep, bytes=nbytes, launches=launches, ms=ms)
+    return dt / steps * 1e3, iters, prof
 
 
-def measured_traffic(kernel, ng, nz, precision):
-    """HBM bytes per launch of the roofline kernel from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE
-    readings (profiles/hbm_traffic_*.txt), scaled by the copy-kernel calibration; None when no
-    reading for this kernel and workload is on file."""
-    path = os.path.join(ROOT, "profiles", "hbm_traffic_psolve_deriv_pass_ng512_nz16.txt")
+def measured_traffic(site, kernel, ng, nz, precision):
+    """HBM bytes per launch of a profiled kernel from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE
+    readings (profiles/hbm_traffic_<site>_ng<ng>_nz<nz>.txt), scaled by the copy-kernel calibration;
+    None when no reading for this kernel and workload is on file."""
+    path = os.path.join(ROOT, "profiles", f"hbm_traffic_{site}_ng{ng}_nz{nz}.txt")
     if not os.path.exists(path):
         return None
     kv = {}
@@ -101,6 +106,12 @@ def measured_traffic(kernel, ng, nz, precision):
     kib = 1024.0
     return (float(kv["fetch_kib_per_dispatch"]) * float(kv["fetch_scale"])
             + float(kv["write_kib_per_dispatch"]) * float(kv["write_scale"])) * kib
+
+
+def kernel_names(ng, precision):
+    t, nw = ("double", 4) if precision == "fp64" else ("float", 8)
+    return {"psolve_deriv": f"fftw_inv_pass<{t}, {ng}, {nw}, true, psolve theta-derivative prologue>",
+            "psolve_comb": f"fftw_inv_comb_fwd<{t}, {ng}, {nw}, psolve source fold>"}
 
 
 def rel_l2(a, b):
@@ -173,9 +184,9 @@ def main():
     res = {}
     for p, name in ((sw.FP64, "fp64"), (sw.MIXED, "mixed")):
         spec, st = new_state(sw, ng, nz, p, init)
-        ms, iters, pms, pbytes, plaunch = time_steps(sw, spec, st, args.steps, args.warmup, sw.PROF_PSOLVE_DERIV)
-        res[name] = dict(ms=ms, iters=iters, fft_ms=pms, fft_bytes=pbytes, fft_launches=plaunch,
-                         value=points / (ms * 1e-3))
+        modes = {"psolve_deriv": sw.PROF_PSOLVE_DERIV, "psolve_comb": sw.PROF_PSOLVE_COMB}
+        ms, iters, prof = time_steps(sw, spec, st, args.steps, args.warmup, modes)
+        res[name] = dict(ms=ms, iters=iters, prof=prof, value=points / (ms * 1e-3))
         print(f"[bench] {name}: {ms:.3f} ms/step, {points / (ms * 1e-3):.4g} point-steps/s, psolve iterations "
               f"per step {iters}", file=sys.stderr, flush=True)
         del st, spec
@@ -186,12 +197,18 @@ def main():
     meets = all(v <= ACCURACY_BAR for v in err.values())
     head = "mixed" if meets else "fp64"
     h, f64, m = res[head], res["fp64"], res["mixed"]
-    achieved = h["fft_bytes"] / (h["fft_ms"] * 1e-3) / 1e9 if h["fft_ms"] > 0 else 0.0
+    # roofline of the dominant profiled kernel (largest share of the step)
+    site = max(h["prof"], key=lambda k: h["prof"][k]["ms_per_step"])
+    pk = h["prof"][site]
+    achieved = pk["bytes"] / (pk["ms"] * 1e-3) / 1e9 if pk["ms"] > 0 else 0.0
+    names = kernel_names(ng, head)
+    kname = names[site]
+    others = {k: {"kernel": names[k], "ms_per_step": v["ms_per_step"], "avg_launch_us": v["ms"] * 1e3 / max(v["launches"], 1),
+                  "achieved_GBs": v["bytes"] / (v["ms"] * 1e-3) / 1e9 if v["ms"] > 0 else 0.0}
+              for k, v in h["prof"].items()}
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(sw, ng, nz, init)
-    kname = {"fp64": "fftw_inv_pass<double, 512, 4, true, psolve theta-derivative prologue>",
-             "mixed": "fftw_inv_pass<float, 512, 8, true, psolve theta-derivative prologue>"}[head]
     line = {
         "metric": METRIC,
         "value": h["value"],
@@ -216,10 +233,11 @@ def main():
                   "dtype": "f32 bulk, f64 psolve tridiagonal+residual and reductions",
                   "over_fp64": m["value"] / f64["value"]},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(kname, ng, nz, head),
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(site, kname, ng, nz, head),
                      "kernel": kname,
-                     "launches": h["fft_launches"], "avg_launch_us": h["fft_ms"] * 1e3 / max(h["fft_launches"], 1),
-                     "algorithmic_bytes_per_launch": h["fft_bytes"] / max(h["fft_launches"], 1)},
+                     "launches": pk["launches"], "avg_launch_us": pk["ms"] * 1e3 / max(pk["launches"], 1),
+                     "algorithmic_bytes_per_launch": pk["bytes"] / max(pk["launches"], 1),
+                     "profiled_kernels": others},
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

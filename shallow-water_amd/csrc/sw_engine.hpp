@@ -10,6 +10,7 @@
 #include <cstring>
 #include <map>
 #include <string>
+#include <type_traits>
 
 #include "sw_common.hpp"
 #include "sw_fft.hpp"
@@ -129,6 +130,23 @@ __global__ void k_meanflow_finalize(const double* part, int nb, int nlev, const 
     out[0] = uio;
     out[1] = vio;
 }
+
+// The pointwise fold of psolve's fused source launch (fftw_inv_comb_fwd): for level s and physical
+// point (y, x), S = sp0 + sigx px + sigy py + cpt2 ptt + cpt1 pt from the level's derivative slabs
+// k = 0..3 (x, y, theta, theta-theta), or sp0 + cpt2 ptt at the bottom level (one slab)
+// (psolve.rs:101-270).
+template <typename T>
+struct PsolveComb {
+    const T* sp0;
+    PtrTab<const T, 4> cf;  // coefficient of slab k: sigx, sigy, cpt1, cpt2
+    int n;
+    size_t nn;
+    __device__ __forceinline__ int count(int s) const { return s == 0 ? 1 : 4; }
+    __device__ __forceinline__ T init(int s, int x, int y) const { return sp0[(size_t)s * nn + y + (size_t)n * x]; }
+    __device__ __forceinline__ T coef(int s, int k, int x, int y) const {
+        return cf[s == 0 ? 3 : k][(size_t)s * nn + y + (size_t)n * x];
+    }
+};
 
 template <typename T>
 struct Engine {
@@ -276,19 +294,34 @@ struct Engine {
     }
 
     // call sites a profiling mode can single out
-    enum { SITE_OTHER = 0, SITE_PSOLVE_DERIV = 1 };
+    enum { SITE_OTHER = 0, SITE_PSOLVE_DERIV = 1, SITE_PSOLVE_COMB = 2 };
 
     struct NoEpi {
         __device__ void operator()(int, int, int, T) const {}
     };
+
+    // Transform lengths with wave-per-sequence kernels (sw_fft_wave.hpp); others use the
+    // block-per-rows kernels of sw_fft.hpp.
+    bool wave_n() const { return ng == 128 || ng == 256 || ng == 512; }
+    template <class F>
+    void wave_dispatch(F f) {
+        if (ng == 512) f(std::integral_constant<int, 512>{});
+        else if (ng == 256) f(std::integral_constant<int, 256>{});
+        else f(std::integral_constant<int, 128>{});
+    }
 
     // Forward 2-D transform of nslab slabs: pro(s, x, y) gives the physical value at (y, x);
     // epi(s, ky, kx, v) receives the spectral coefficient at (kx, ky).
     template <class Pro, class Epi>
     void fwd2d(int nslab, Pro pro, Epi epi, int site = SITE_OTHER) {
         if (nslab <= 0) return;
-        if (ng == 512) {
-            fwd2d_wave<512>(nslab, pro, epi, site);
+        if (wave_n()) {
+            T* t = tmp_for(nslab);
+            wave_dispatch([&](auto nn) {
+                constexpr int NN = decltype(nn)::value;
+                fwd_pass1_wave<NN>(nslab, pro, t, site);
+                fwd_pass2_wave<NN>(nslab, t, epi);
+            });
             return;
         }
         T* t = tmp_for(nslab);
@@ -312,8 +345,13 @@ struct Engine {
     template <class Pro, class Epi>
     void inv2d(int nslab, Pro pro, Epi epi, int site = SITE_OTHER) {
         if (nslab <= 0) return;
-        if (ng == 512) {
-            inv2d_wave<512>(nslab, pro, epi, site);
+        if (wave_n()) {
+            T* t = tmp_for(nslab);
+            wave_dispatch([&](auto nn) {
+                constexpr int NN = decltype(nn)::value;
+                inv_pass1_wave<NN>(nslab, pro, t, site);
+                inv_pass2_wave<NN>(nslab, t, epi);
+            });
             return;
         }
         T* t = tmp_for(nslab);
@@ -332,44 +370,68 @@ struct Engine {
         SW_HIP(hipGetLastError());
     }
 
-    // The same two passes with the wave-per-sequence kernels (sw_fft_wave.hpp) for n = NN.  The
-    // transposing pass groups NWT waves so that its store writes 64-byte runs; the straight pass runs
-    // one wave per block.
+    // Single passes of the wave-per-sequence kernels for n = NN.  A transposing pass groups NWT waves so
+    // that its store writes 2 NWT consecutive values (64 bytes FP64, 64 bytes FP32) per output row; a
+    // straight pass runs one wave per block.  Pass 1 of a forward transform runs along y (physical rows
+    // x) and leaves the spectrum in y transposed: out[x + n ky]; pass 2 runs along x.  Pass 1 of an
+    // inverse transform runs along kx (rows ky) and leaves out[ky + n x]; pass 2 runs along ky.
     static constexpr int NWT = 32 / (int)sizeof(T);
-    template <int NN, class Pro, class Epi>
-    void fwd2d_wave(int nslab, Pro pro, Epi epi, int site) {
-        T* t = tmp_for(nslab);
-        const double bytes = 2.0 * sizeof(T) * (double)n2 * nslab;
-        const bool time1 = prof == SW_PROF_PSOLVE_DERIV && site == SITE_PSOLVE_DERIV, time2 = prof == SW_PROF_STRAIGHT;
-        const size_t lds1 = wave_lds_bytes<NN, NWT, T>(), lds2 = wave_lds_bytes<NN, 1, T>();
-        if (time1) prof_start(bytes);
+    bool timed(int site) const {
+        return (prof == SW_PROF_PSOLVE_DERIV && site == SITE_PSOLVE_DERIV) ||
+               (prof == SW_PROF_PSOLVE_COMB && site == SITE_PSOLVE_COMB);
+    }
+    template <int NN, class Pro>
+    void fwd_pass1_wave(int nslab, Pro pro, T* out, int site) {
+        const size_t lds_t = wave_lds_bytes<NN, NWT, T>();
+        const bool tm = timed(site);
+        if (tm) prof_start(2.0 * sizeof(T) * (double)n2 * nslab);
         hipLaunchKernelGGL((fftw_fwd_pass<T, NN, NWT, true, Pro, NoEpi>), dim3(nslab * (NN / (2 * NWT))), dim3(64 * NWT),
-                           lds1, st, plan.tw, nslab, pro, NoEpi{}, t);
-        if (time1) prof_stop();
-        SW_HIP(hipGetLastError());
-        LoadPlain<T> lp{t, n2, ng};
-        if (time2) prof_start(bytes);
-        hipLaunchKernelGGL((fftw_fwd_pass<T, NN, 1, false, LoadPlain<T>, Epi>), dim3(nslab * (NN / 2)), dim3(64), lds2, st,
-                           plan.tw, nslab, lp, epi, (T*)nullptr);
-        if (time2) prof_stop();
+                           lds_t, st, plan.tw, nslab, pro, NoEpi{}, out);
+        if (tm) prof_stop();
         SW_HIP(hipGetLastError());
     }
-    template <int NN, class Pro, class Epi>
-    void inv2d_wave(int nslab, Pro pro, Epi epi, int site) {
-        T* t = tmp_for(nslab);
-        const double bytes = 2.0 * sizeof(T) * (double)n2 * nslab;
-        const bool time1 = prof == SW_PROF_PSOLVE_DERIV && site == SITE_PSOLVE_DERIV, time2 = prof == SW_PROF_STRAIGHT;
-        const size_t lds1 = wave_lds_bytes<NN, NWT, T>(), lds2 = wave_lds_bytes<NN, 1, T>();
-        if (time1) prof_start(bytes);
-        hipLaunchKernelGGL((fftw_inv_pass<T, NN, NWT, true, Pro, NoEpi>), dim3(nslab * (NN / (2 * NWT))), dim3(64 * NWT),
-                           lds1, st, plan.tw, nslab, pro, NoEpi{}, t);
-        if (time1) prof_stop();
+    template <int NN, class Epi>
+    void fwd_pass2_wave(int nslab, const T* in, Epi epi) {
+        const size_t lds_s = wave_lds_bytes<NN, 1, T>();
+        const bool tm = prof == SW_PROF_STRAIGHT;
+        if (tm) prof_start(2.0 * sizeof(T) * (double)n2 * nslab);
+        LoadPlain<T> lp{in, n2, ng};
+        hipLaunchKernelGGL((fftw_fwd_pass<T, NN, 1, false, LoadPlain<T>, Epi>), dim3(nslab * (NN / 2)), dim3(64),
+                           lds_s, st, plan.tw, nslab, lp, epi, (T*)nullptr);
+        if (tm) prof_stop();
         SW_HIP(hipGetLastError());
-        LoadPlain<T> lp{t, n2, ng};
-        if (time2) prof_start(bytes);
-        hipLaunchKernelGGL((fftw_inv_pass<T, NN, 1, false, LoadPlain<T>, Epi>), dim3(nslab * (NN / 2)), dim3(64), lds2, st,
-                           plan.tw, nslab, lp, epi, (T*)nullptr);
-        if (time2) prof_stop();
+    }
+    template <int NN, class Pro>
+    void inv_pass1_wave(int nslab, Pro pro, T* out, int site) {
+        const size_t lds_t = wave_lds_bytes<NN, NWT, T>();
+        const bool tm = timed(site);
+        if (tm) prof_start(2.0 * sizeof(T) * (double)n2 * nslab);
+        hipLaunchKernelGGL((fftw_inv_pass<T, NN, NWT, true, Pro, NoEpi>), dim3(nslab * (NN / (2 * NWT))), dim3(64 * NWT),
+                           lds_t, st, plan.tw, nslab, pro, NoEpi{}, out);
+        if (tm) prof_stop();
+        SW_HIP(hipGetLastError());
+    }
+    template <int NN, class Epi>
+    void inv_pass2_wave(int nslab, const T* in, Epi epi) {
+        const size_t lds_s = wave_lds_bytes<NN, 1, T>();
+        const bool tm = prof == SW_PROF_STRAIGHT;
+        if (tm) prof_start(2.0 * sizeof(T) * (double)n2 * nslab);
+        LoadPlain<T> lp{in, n2, ng};
+        hipLaunchKernelGGL((fftw_inv_pass<T, NN, 1, false, LoadPlain<T>, Epi>), dim3(nslab * (NN / 2)), dim3(64),
+                           lds_s, st, plan.tw, nslab, lp, epi, (T*)nullptr);
+        if (tm) prof_stop();
+        SW_HIP(hipGetLastError());
+    }
+    // fftw_inv_comb_fwd over nslab slab groups (transposed output); bytes = the algorithmic bytes of
+    // the launch, for the profiling mode that times this site.
+    template <int NN, class Pro, class Comb>
+    void inv_comb_fwd_wave(int nslab, Pro pro, Comb comb, T* out, double bytes, int site) {
+        const size_t lds_t = wave_lds_bytes<NN, NWT, T>();
+        const bool tm = timed(site);
+        if (tm) prof_start(bytes);
+        hipLaunchKernelGGL((fftw_inv_comb_fwd<T, NN, NWT, Pro, Comb>), dim3(nslab * (NN / (2 * NWT))), dim3(64 * NWT),
+                           lds_t, st, plan.tw, nslab, pro, comb, out);
+        if (tm) prof_stop();
         SW_HIP(hipGetLastError());
     }
 

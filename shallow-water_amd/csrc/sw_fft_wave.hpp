@@ -35,6 +35,7 @@ struct WaveShape<512> {
 // over the banks)
 template <int N>
 constexpr int wave_region() { return N + N / 8; }
+
 __device__ __forceinline__ int wpad(int i) { return i + (i >> 3); }
 
 template <int N, int NW, typename T>
@@ -50,6 +51,11 @@ template <int N, bool INV, typename T>
 __device__ __forceinline__ void wave_stockham(C2<T> (&v)[WaveShape<N>::E], C2<T>* buf, const C2<T>* __restrict__ tw,
                                               int lane) {
     constexpr int E = WaveShape<N>::E, P = WaveShape<N>::P;
+    // The twiddle loads stay next to their use: an empty asm makes the table address opaque, so the
+    // compiler cannot hoist the loads out of a caller's loop and hold (E - 1) (P - 1) complex values in
+    // registers for the whole kernel.
+    const C2<T>* twp = tw;
+    asm volatile("" : "+s"(twp));
     int Ns = 1;
 #pragma unroll
     for (int q = 0; q < P; q++) {
@@ -58,7 +64,7 @@ __device__ __forceinline__ void wave_stockham(C2<T> (&v)[WaveShape<N>::E], C2<T>
             const int stride = N / (Ns * E);
 #pragma unroll
             for (int r = 1; r < E; r++) {
-                C2<T> w = tw[r * k * stride];
+                C2<T> w = twp[r * k * stride];
                 if (INV) w.y = -w.y;
                 v[r] = cmul(v[r], w);
             }
@@ -76,12 +82,115 @@ __device__ __forceinline__ void wave_stockham(C2<T> (&v)[WaveShape<N>::E], C2<T>
     }
 }
 
+// The cosine (a, b) and sine (a', b') coefficients of two Hermitian rows at k = lane + 64 t <= N/2,
+// fetched into registers ahead of wave_pack_herm; ld(h, f) gives row h (0 or 1) at packed position f.
+template <int N, typename T>
+struct HermPair {
+    static constexpr int NT = WaveShape<N>::E / 2 + 1;
+    T ca[NT], cb[NT], sa[NT], sb[NT];
+};
+template <int N, typename T, class Ld>
+__device__ __forceinline__ void wave_fetch_herm(HermPair<N, T>& h, int lane, Ld ld) {
+    constexpr int NT = HermPair<N, T>::NT;
+#pragma unroll
+    for (int t = 0; t < NT; t++) {
+        const int k = lane + 64 * t;
+        if (t < NT - 1 || lane == 0) {
+            const bool has_s = k > 0 && 2 * k < N;
+            h.ca[t] = ld(0, k);
+            h.cb[t] = ld(1, k);
+            h.sa[t] = has_s ? ld(0, N - k) : T(0);
+            h.sb[t] = has_s ? ld(1, N - k) : T(0);
+        }
+    }
+}
+// Pack fetched rows as one complex sequence ready for the inverse transform, Z_k = (a_k - b'_k,
+// a'_k + b_k), Z_{N-k} = (a_k + b'_k, b_k - a'_k), through this wave's LDS region into v.
+template <int N, typename T>
+__device__ __forceinline__ void wave_pack_herm(C2<T> (&v)[WaveShape<N>::E], C2<T>* buf, int lane, const HermPair<N, T>& h) {
+    constexpr int E = WaveShape<N>::E, NT = HermPair<N, T>::NT;
+    __syncthreads();  // every lane has read the previous contents of buf
+#pragma unroll
+    for (int t = 0; t < NT; t++) {
+        const int k = lane + 64 * t;
+        if (t < NT - 1 || lane == 0) {
+            buf[wpad(k)] = C2<T>{h.ca[t] - h.sb[t], h.sa[t] + h.cb[t]};
+            if (k > 0 && 2 * k < N) buf[wpad(N - k)] = C2<T>{h.ca[t] + h.sb[t], h.cb[t] - h.sa[t]};
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < E; r++) v[r] = buf[wpad(lane + 64 * r)];
+}
+template <int N, typename T, class Ld>
+__device__ __forceinline__ void wave_load_herm_pair(C2<T> (&v)[WaveShape<N>::E], C2<T>* buf, int lane, Ld ld) {
+    HermPair<N, T> h;
+    wave_fetch_herm<N>(h, lane, ld);
+    wave_pack_herm<N>(v, buf, lane, h);
+}
+
+// After the forward transform of a packed pair (v[r] = Z[lane + 64 r]), split the two real transforms,
+// A_k = (Z_k + conj Z_{N-k}) / 2, B_k = (Z_k - conj Z_{N-k}) / 2i, scaled by 1/sqrt(N): cosine parts
+// ar, br and sine parts ai, bi at k = lane + 64 t <= N/2 (t < E/2 + 1; the last t only on lane 0).
+template <int N, typename T>
+__device__ __forceinline__ void wave_split_pair(const C2<T> (&v)[WaveShape<N>::E], C2<T>* buf, int lane,
+                                                T (&ar)[WaveShape<N>::E / 2 + 1], T (&ai)[WaveShape<N>::E / 2 + 1],
+                                                T (&br)[WaveShape<N>::E / 2 + 1], T (&bi)[WaveShape<N>::E / 2 + 1]) {
+    constexpr int E = WaveShape<N>::E, NT = E / 2 + 1;
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < E; r++) buf[wpad(lane + 64 * r)] = v[r];
+    __syncthreads();
+    const T sc = T(0.5) / sqrt(T(N));
+#pragma unroll
+    for (int t = 0; t < NT; t++) {
+        const int k = lane + 64 * t;
+        if (t < NT - 1 || lane == 0) {
+            const int kc = (N - k) & (N - 1);
+            C2<T> z = buf[wpad(k)], w = buf[wpad(kc)];
+            ar[t] = (z.x + w.x) * sc;
+            ai[t] = (z.y - w.y) * sc;
+            br[t] = (z.y + w.y) * sc;
+            bi[t] = (w.x - z.x) * sc;
+        }
+    }
+}
+
+// Transposed store of the 2 NW rows r0 .. r0 + 2 NW - 1 of a block (wave wv holds rows r0 + 2 wv and
+// r0 + 2 wv + 1 as split by wave_split_pair): element (r, f) goes to o[r + N f], written through an LDS
+// stage [f][row] so that each output row gets a run of 2 NW consecutive values.  o = slab base + r0.
+template <int N, int NW, typename T>
+__device__ __forceinline__ void wave_store_trans(const T (&ar)[WaveShape<N>::E / 2 + 1], const T (&ai)[WaveShape<N>::E / 2 + 1],
+                                                 const T (&br)[WaveShape<N>::E / 2 + 1], const T (&bi)[WaveShape<N>::E / 2 + 1],
+                                                 T* stg, int lane, int wv, T* o) {
+    constexpr int E = WaveShape<N>::E, NT = E / 2 + 1, RB = 2 * NW;
+    __syncthreads();
+#pragma unroll
+    for (int t = 0; t < NT; t++) {
+        const int k = lane + 64 * t;
+        if (t < NT - 1 || lane == 0) {
+            stg[k * RB + 2 * wv] = ar[t];
+            stg[k * RB + 2 * wv + 1] = br[t];
+            if (k > 0 && 2 * k < N) {
+                stg[(N - k) * RB + 2 * wv] = ai[t];
+                stg[(N - k) * RB + 2 * wv + 1] = bi[t];
+            }
+        }
+    }
+    __syncthreads();
+#pragma unroll 4
+    for (int e = threadIdx.x; e < RB * N; e += 64 * NW) {
+        const int f = e / RB, rr = e - f * RB;
+        o[(size_t)f * N + rr] = stg[e];
+    }
+}
+
 // Forward (real -> Hermitian) pass; pro(s, r, j) gives row r, element j.  TRANS: element (r, f) goes
 // to out[r + n f + n n s]; otherwise epi(s, r, f, value).
 template <typename T, int N, int NW, bool TRANS, class Pro, class Epi>
 __global__ void __launch_bounds__(64 * NW) fftw_fwd_pass(const C2<T>* __restrict__ tw, int nslab, Pro pro, Epi epi,
                                                         T* tout) {
-    constexpr int E = WaveShape<N>::E, BPS = N / (2 * NW);
+    constexpr int E = WaveShape<N>::E, BPS = N / (2 * NW), NT = E / 2 + 1;
     extern __shared__ __align__(16) unsigned char lds_raw[];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     C2<T>* buf = reinterpret_cast<C2<T>*>(lds_raw) + wv * wave_region<N>();
@@ -95,27 +204,8 @@ __global__ void __launch_bounds__(64 * NW) fftw_fwd_pass(const C2<T>* __restrict
         v[r] = C2<T>{pro(s, ra, j), pro(s, ra + 1, j)};
     }
     wave_stockham<N, false>(v, buf, tw, lane);
-
-    // split the two real transforms: A_k = (Z_k + conj Z_{N-k}) / 2, B_k = (Z_k - conj Z_{N-k}) / 2i
-    __syncthreads();
-#pragma unroll
-    for (int r = 0; r < E; r++) buf[wpad(lane + 64 * r)] = v[r];
-    __syncthreads();
-    const T sc = T(0.5) / sqrt(T(N));
-    constexpr int NT = E / 2 + 1;  // k = lane + 64 t <= N/2
     T ar[NT], ai[NT], br[NT], bi[NT];
-#pragma unroll
-    for (int t = 0; t < NT; t++) {
-        const int k = lane + 64 * t;
-        if (t < NT - 1 || lane == 0) {
-            const int kc = (N - k) & (N - 1);
-            C2<T> z = buf[wpad(k)], w = buf[wpad(kc)];
-            ar[t] = (z.x + w.x) * sc;
-            ai[t] = (z.y - w.y) * sc;
-            br[t] = (z.y + w.y) * sc;
-            bi[t] = (w.x - z.x) * sc;
-        }
-    }
+    wave_split_pair<N>(v, buf, lane, ar, ai, br, bi);
     if (!TRANS) {
 #pragma unroll
         for (int t = 0; t < NT; t++) {
@@ -130,28 +220,7 @@ __global__ void __launch_bounds__(64 * NW) fftw_fwd_pass(const C2<T>* __restrict
             }
         }
     } else {
-        constexpr int RB = 2 * NW;  // rows of the block
-        T* stg = reinterpret_cast<T*>(lds_raw);  // [f][row]
-        __syncthreads();
-#pragma unroll
-        for (int t = 0; t < NT; t++) {
-            const int k = lane + 64 * t;
-            if (t < NT - 1 || lane == 0) {
-                stg[k * RB + 2 * wv] = ar[t];
-                stg[k * RB + 2 * wv + 1] = br[t];
-                if (k > 0 && 2 * k < N) {
-                    stg[(N - k) * RB + 2 * wv] = ai[t];
-                    stg[(N - k) * RB + 2 * wv + 1] = bi[t];
-                }
-            }
-        }
-        __syncthreads();
-        T* o = tout + (size_t)s * N * N + r0;
-#pragma unroll 4
-        for (int e = threadIdx.x; e < RB * N; e += 64 * NW) {
-            const int f = e / RB, rr = e - f * RB;
-            o[(size_t)f * N + rr] = stg[e];
-        }
+        wave_store_trans<N, NW>(ar, ai, br, bi, reinterpret_cast<T*>(lds_raw), lane, wv, tout + (size_t)s * N * N + r0);
     }
 }
 
@@ -167,24 +236,8 @@ __global__ void __launch_bounds__(64 * NW) fftw_inv_pass(const C2<T>* __restrict
     const int s = blockIdx.x / BPS, r0 = (blockIdx.x - s * BPS) * 2 * NW, ra = r0 + 2 * wv;
     if (s >= nslab) return;
 
-    // pack: Z_k = (a_k - b'_k, a'_k + b_k) and Z_{N-k} = (a_k + b'_k, b_k - a'_k) from the cosine (a, b)
-    // and sine (a', b') coefficients of the two rows at k <= N/2
-    constexpr int NT = E / 2 + 1;
-#pragma unroll
-    for (int t = 0; t < NT; t++) {
-        const int k = lane + 64 * t;
-        if (t < NT - 1 || lane == 0) {
-            const bool has_s = k > 0 && 2 * k < N;
-            const T ca = pro(s, ra, k), cb = pro(s, ra + 1, k);
-            const T sa = has_s ? pro(s, ra, N - k) : T(0), sb = has_s ? pro(s, ra + 1, N - k) : T(0);
-            buf[wpad(k)] = C2<T>{ca - sb, sa + cb};
-            if (has_s) buf[wpad(N - k)] = C2<T>{ca + sb, cb - sa};
-        }
-    }
-    __syncthreads();
     C2<T> v[E];
-#pragma unroll
-    for (int r = 0; r < E; r++) v[r] = buf[wpad(lane + 64 * r)];
+    wave_load_herm_pair<N>(v, buf, lane, [&](int h, int f) { return pro(s, ra + h, f); });
     wave_stockham<N, true>(v, buf, tw, lane);
 
     const T sc = T(1) / sqrt(T(N));
@@ -213,6 +266,58 @@ __global__ void __launch_bounds__(64 * NW) fftw_inv_pass(const C2<T>* __restrict
             o[(size_t)x * N + rr] = stg[e];
         }
     }
+}
+
+// Inverse pass, pointwise combination and forward pass along the same axis in one launch.  For slab
+// group s (block-uniform) the block's rows r are the inverse transforms of K = comb.count(s) Hermitian
+// inputs, pro(s, k, r, f) = input k, row r, packed position f; each transformed value val (row r,
+// position j) is folded into a per-element accumulator, a = a + comb.coef(s, k, r, j) * val, which
+// starts from comb.init(s, r, j); the accumulated real row is transformed forward again and stored
+// transposed: element (r, f) at out[r + n f + n n s].  Nothing between the two transforms goes
+// through HBM.
+template <typename T, int N, int NW, class Pro, class Comb>
+__global__ void __launch_bounds__(64 * NW) fftw_inv_comb_fwd(const C2<T>* __restrict__ tw, int nslab, Pro pro, Comb comb, T* tout) {
+    constexpr int E = WaveShape<N>::E, BPS = N / (2 * NW), NT = E / 2 + 1;
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    C2<T>* buf = reinterpret_cast<C2<T>*>(lds_raw) + wv * wave_region<N>();
+    const int s = blockIdx.x / BPS, r0 = (blockIdx.x - s * BPS) * 2 * NW, ra = r0 + 2 * wv;
+    if (s >= nslab) return;
+
+    C2<T> acc[E];  // rows ra (.x) and ra + 1 (.y) at j = lane + 64 r
+#pragma unroll
+    for (int r = 0; r < E; r++) {
+        const int j = lane + 64 * r;
+        acc[r] = C2<T>{comb.init(s, ra, j), comb.init(s, ra + 1, j)};
+    }
+    const int K = comb.count(s);
+    const T sc = T(1) / sqrt(T(N));
+    // software pipeline: input k + 1 and the coefficients of input k are in flight during the
+    // transform of input k
+    HermPair<N, T> h;
+    wave_fetch_herm<N>(h, lane, [&](int hh, int f) { return pro(s, 0, ra + hh, f); });
+    C2<T> v[E];
+#pragma unroll 1
+    for (int k = 0; k < K; k++) {
+        wave_pack_herm<N>(v, buf, lane, h);
+        if (k + 1 < K) wave_fetch_herm<N>(h, lane, [&](int hh, int f) { return pro(s, k + 1, ra + hh, f); });
+        C2<T> cw[E];
+#pragma unroll
+        for (int r = 0; r < E; r++) {
+            const int j = lane + 64 * r;
+            cw[r] = C2<T>{comb.coef(s, k, ra, j), comb.coef(s, k, ra + 1, j)};
+        }
+        wave_stockham<N, true>(v, buf, tw, lane);
+#pragma unroll
+        for (int r = 0; r < E; r++) {
+            acc[r].x = acc[r].x + cw[r].x * (v[r].x * sc);
+            acc[r].y = acc[r].y + cw[r].y * (v[r].y * sc);
+        }
+    }
+    wave_stockham<N, false>(acc, buf, tw, lane);
+    T ar[NT], ai[NT], br[NT], bi[NT];
+    wave_split_pair<N>(acc, buf, lane, ar, ai, br, bi);
+    wave_store_trans<N, NW>(ar, ai, br, bi, reinterpret_cast<T*>(lds_raw), lane, wv, tout + (size_t)s * N * N + r0);
 }
 
 }  // namespace sw

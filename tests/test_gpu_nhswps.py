@@ -113,7 +113,7 @@ def synthetic(sw, oracle, ng, nz, seed):
     return g, o
 
 
-@pytest.mark.parametrize("ng,nz", [(64, 4), (96, 6), (512, 4)])
+@pytest.mark.parametrize("ng,nz", [(64, 4), (96, 6), (128, 4), (256, 8), (512, 4)])
 def test_one_step_synthetic(sw, oracle, ng, nz):
     g, o = synthetic(sw, oracle, ng, nz, seed=1)
     diag, iters = sw.advance(g)
