@@ -114,34 +114,40 @@ __global__ void __launch_bounds__(256) k_level_sum2(size_t n2, F f, double* part
 }
 
 // Mean flow of main_invert (spectral/mod.rs:409-416): out[c] = -sum_iz w_iz * S_iz,c / ng^2, with
-// the level sums S taken in a fixed order from the partials (deterministic).
-__global__ void k_meanflow_finalize(const double* part, int nb, int nlev, const double* w, double dsumi, double* out) {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+// the level sums S taken in a fixed order from the partials (deterministic): one thread per (level,
+// component) sums the nb block partials in block order, then the levels are combined in order.
+__global__ void __launch_bounds__(256) k_meanflow_finalize(const double* part, int nb, int nlev, const double* w,
+                                                           double dsumi, double* out) {
+    __shared__ double ls[2 * (MAX_NZ + 1)];
+    for (int t = threadIdx.x; t < 2 * nlev; t += blockDim.x) {
+        const int iz = t >> 1, c = t & 1;
+        double s = 0.0;
+        for (int b = 0; b < nb; b++) s += part[((size_t)iz * nb + b) * 2 + c];
+        ls[t] = s;
+    }
+    __syncthreads();
+    if (threadIdx.x != 0) return;
     double uio = 0.0, vio = 0.0;
     for (int iz = 0; iz < nlev; iz++) {
-        double s0 = 0.0, s1 = 0.0;
-        for (int b = 0; b < nb; b++) {
-            s0 += part[((size_t)iz * nb + b) * 2];
-            s1 += part[((size_t)iz * nb + b) * 2 + 1];
-        }
-        uio -= w[iz] * s0 * dsumi;
-        vio -= w[iz] * s1 * dsumi;
+        uio -= w[iz] * ls[2 * iz] * dsumi;
+        vio -= w[iz] * ls[2 * iz + 1] * dsumi;
     }
     out[0] = uio;
     out[1] = vio;
 }
 
 // The pointwise fold of psolve's fused source launch (fftw_inv_comb_fwd): for level s and physical
-// point (y, x), S = sp0 + sigx px + sigy py + cpt2 ptt + cpt1 pt from the level's derivative slabs
-// k = 0..3 (x, y, theta, theta-theta), or sp0 + cpt2 ptt at the bottom level (one slab)
-// (psolve.rs:101-270).
+// point (y, x), S = sp0 + sigx px + cpt1 pt + sigy py + cpt2 ptt from the level's derivative inputs
+// k = 0..3 (x, theta, y, theta-theta; y is the derivative of the theta input along the transform
+// axis), or sp0 + cpt2 ptt at the bottom level (one input) (psolve.rs:101-270).
 template <typename T>
 struct PsolveComb {
     const T* sp0;
-    PtrTab<const T, 4> cf;  // coefficient of slab k: sigx, sigy, cpt1, cpt2
+    PtrTab<const T, 4> cf;  // coefficient of input k: sigx, cpt1, sigy, cpt2
     int n;
     size_t nn;
     __device__ __forceinline__ int count(int s) const { return s == 0 ? 1 : 4; }
+    __device__ __forceinline__ bool deriv_of_prev(int s, int k) const { return s > 0 && k == 2; }
     __device__ __forceinline__ T init(int s, int x, int y) const { return sp0[(size_t)s * nn + y + (size_t)n * x]; }
     __device__ __forceinline__ T coef(int s, int k, int x, int y) const {
         return cf[s == 0 ? 3 : k][(size_t)s * nn + y + (size_t)n * x];
@@ -310,16 +316,31 @@ struct Engine {
         else f(std::integral_constant<int, 128>{});
     }
 
+    // f(integral_constant<int, nz>) for the level counts with a compile-time specialisation, else
+    // f(integral_constant<int, 0>).
+    template <class F>
+    void nz_dispatch(F f) {
+        switch (nz) {
+            case 4: f(std::integral_constant<int, 4>{}); break;
+            case 8: f(std::integral_constant<int, 8>{}); break;
+            case 16: f(std::integral_constant<int, 16>{}); break;
+            case 24: f(std::integral_constant<int, 24>{}); break;
+            case 32: f(std::integral_constant<int, 32>{}); break;
+            default: f(std::integral_constant<int, 0>{}); break;
+        }
+    }
+
     // Forward 2-D transform of nslab slabs: pro(s, x, y) gives the physical value at (y, x);
     // epi(s, ky, kx, v) receives the spectral coefficient at (kx, ky).
-    template <class Pro, class Epi>
+    // NW: waves per block of the transposing pass (0: NWT).
+    template <int NW = 0, class Pro, class Epi>
     void fwd2d(int nslab, Pro pro, Epi epi, int site = SITE_OTHER) {
         if (nslab <= 0) return;
         if (wave_n()) {
             T* t = tmp_for(nslab);
             wave_dispatch([&](auto nn) {
                 constexpr int NN = decltype(nn)::value;
-                fwd_pass1_wave<NN>(nslab, pro, t, site);
+                fwd_pass1_wave<NN, NW ? NW : NWT>(nslab, pro, t, site);
                 fwd_pass2_wave<NN>(nslab, t, epi);
             });
             return;
@@ -342,14 +363,14 @@ struct Engine {
 
     // Inverse 2-D transform: pro(s, ky, kx) gives the spectral coefficient at (kx, ky);
     // epi(s, x, y, v) receives the physical value at (y, x).
-    template <class Pro, class Epi>
+    template <int NW = 0, class Pro, class Epi>
     void inv2d(int nslab, Pro pro, Epi epi, int site = SITE_OTHER) {
         if (nslab <= 0) return;
         if (wave_n()) {
             T* t = tmp_for(nslab);
             wave_dispatch([&](auto nn) {
                 constexpr int NN = decltype(nn)::value;
-                inv_pass1_wave<NN>(nslab, pro, t, site);
+                inv_pass1_wave<NN, NW ? NW : NWT>(nslab, pro, t, site);
                 inv_pass2_wave<NN>(nslab, t, epi);
             });
             return;
@@ -370,22 +391,22 @@ struct Engine {
         SW_HIP(hipGetLastError());
     }
 
-    // Single passes of the wave-per-sequence kernels for n = NN.  A transposing pass groups NWT waves so
-    // that its store writes 2 NWT consecutive values (64 bytes FP64, 64 bytes FP32) per output row; a
-    // straight pass runs one wave per block.  Pass 1 of a forward transform runs along y (physical rows
+    // Single passes of the wave-per-sequence kernels for n = NN.  A transposing pass groups NW waves
+    // (NWT by default) so that its store writes 2 NW consecutive values per output row (128 bytes FP64,
+    // 64 bytes FP32); a straight pass runs one wave per block.  Pass 1 of a forward transform runs along y (physical rows
     // x) and leaves the spectrum in y transposed: out[x + n ky]; pass 2 runs along x.  Pass 1 of an
     // inverse transform runs along kx (rows ky) and leaves out[ky + n x]; pass 2 runs along ky.
-    static constexpr int NWT = 32 / (int)sizeof(T);
+    static constexpr int NWT = 8;
     bool timed(int site) const {
         return (prof == SW_PROF_PSOLVE_DERIV && site == SITE_PSOLVE_DERIV) ||
                (prof == SW_PROF_PSOLVE_COMB && site == SITE_PSOLVE_COMB);
     }
-    template <int NN, class Pro>
+    template <int NN, int NW, class Pro>
     void fwd_pass1_wave(int nslab, Pro pro, T* out, int site) {
-        const size_t lds_t = wave_lds_bytes<NN, NWT, T>();
+        const size_t lds_t = wave_lds_bytes<NN, NW, T>();
         const bool tm = timed(site);
         if (tm) prof_start(2.0 * sizeof(T) * (double)n2 * nslab);
-        hipLaunchKernelGGL((fftw_fwd_pass<T, NN, NWT, true, Pro, NoEpi>), dim3(nslab * (NN / (2 * NWT))), dim3(64 * NWT),
+        hipLaunchKernelGGL((fftw_fwd_pass<T, NN, NW, true, Pro, NoEpi>), dim3(nslab * (NN / (2 * NW))), dim3(64 * NW),
                            lds_t, st, plan.tw, nslab, pro, NoEpi{}, out);
         if (tm) prof_stop();
         SW_HIP(hipGetLastError());
@@ -401,12 +422,12 @@ struct Engine {
         if (tm) prof_stop();
         SW_HIP(hipGetLastError());
     }
-    template <int NN, class Pro>
+    template <int NN, int NW, class Pro>
     void inv_pass1_wave(int nslab, Pro pro, T* out, int site) {
-        const size_t lds_t = wave_lds_bytes<NN, NWT, T>();
+        const size_t lds_t = wave_lds_bytes<NN, NW, T>();
         const bool tm = timed(site);
         if (tm) prof_start(2.0 * sizeof(T) * (double)n2 * nslab);
-        hipLaunchKernelGGL((fftw_inv_pass<T, NN, NWT, true, Pro, NoEpi>), dim3(nslab * (NN / (2 * NWT))), dim3(64 * NWT),
+        hipLaunchKernelGGL((fftw_inv_pass<T, NN, NW, true, Pro, NoEpi>), dim3(nslab * (NN / (2 * NW))), dim3(64 * NW),
                            lds_t, st, plan.tw, nslab, pro, NoEpi{}, out);
         if (tm) prof_stop();
         SW_HIP(hipGetLastError());

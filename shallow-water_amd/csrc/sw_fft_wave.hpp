@@ -38,6 +38,15 @@ constexpr int wave_region() { return N + N / 8; }
 
 __device__ __forceinline__ int wpad(int i) { return i + (i >> 3); }
 
+// Ordering of LDS accesses among the lanes of one wave (each wave's exchange region is private): a
+// wave's LDS instructions execute in issue order, so only the compiler must be kept from moving memory
+// operations across this point (the fences at wavefront scope), no s_barrier is needed.
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 template <int N, int NW, typename T>
 constexpr size_t wave_lds_bytes() {
     size_t ex = (size_t)NW * wave_region<N>() * sizeof(C2<T>);
@@ -71,10 +80,10 @@ __device__ __forceinline__ void wave_stockham(C2<T> (&v)[WaveShape<N>::E], C2<T>
         }
         dftR<E, INV>(v);
         if (q < P - 1) {
-            __syncthreads();  // every lane has read the previous contents of buf
+            wave_lds_sync();  // every lane has read the previous contents of buf
 #pragma unroll
             for (int r = 0; r < E; r++) buf[wpad((lane - k) * E + k + r * Ns)] = v[r];
-            __syncthreads();
+            wave_lds_sync();
 #pragma unroll
             for (int r = 0; r < E; r++) v[r] = buf[wpad(lane + 64 * r)];
         }
@@ -105,20 +114,30 @@ __device__ __forceinline__ void wave_fetch_herm(HermPair<N, T>& h, int lane, Ld 
     }
 }
 // Pack fetched rows as one complex sequence ready for the inverse transform, Z_k = (a_k - b'_k,
-// a'_k + b_k), Z_{N-k} = (a_k + b'_k, b_k - a'_k), through this wave's LDS region into v.
-template <int N, typename T>
+// a'_k + b_k), Z_{N-k} = (a_k + b'_k, b_k - a'_k), through this wave's LDS region into v.  DERIV: pack
+// the derivative along the transform axis of the fetched rows instead (d[f] = dmul(f) x[mir(f)]: -k
+// times the sine part at k, k times the cosine part at N - k, zero at k = 0 and N/2).
+template <int N, bool DERIV = false, typename T>
 __device__ __forceinline__ void wave_pack_herm(C2<T> (&v)[WaveShape<N>::E], C2<T>* buf, int lane, const HermPair<N, T>& h) {
     constexpr int E = WaveShape<N>::E, NT = HermPair<N, T>::NT;
-    __syncthreads();  // every lane has read the previous contents of buf
+    wave_lds_sync();  // every lane has read the previous contents of buf
 #pragma unroll
     for (int t = 0; t < NT; t++) {
         const int k = lane + 64 * t;
         if (t < NT - 1 || lane == 0) {
-            buf[wpad(k)] = C2<T>{h.ca[t] - h.sb[t], h.sa[t] + h.cb[t]};
-            if (k > 0 && 2 * k < N) buf[wpad(N - k)] = C2<T>{h.ca[t] + h.sb[t], h.cb[t] - h.sa[t]};
+            T ca = h.ca[t], cb = h.cb[t], sa = h.sa[t], sb = h.sb[t];
+            if (DERIV) {
+                const T m = (k == 0 || 2 * k == N) ? T(0) : T(k);
+                ca = -m * h.sa[t];
+                cb = -m * h.sb[t];
+                sa = m * h.ca[t];
+                sb = m * h.cb[t];
+            }
+            buf[wpad(k)] = C2<T>{ca - sb, sa + cb};
+            if (k > 0 && 2 * k < N) buf[wpad(N - k)] = C2<T>{ca + sb, cb - sa};
         }
     }
-    __syncthreads();
+    wave_lds_sync();
 #pragma unroll
     for (int r = 0; r < E; r++) v[r] = buf[wpad(lane + 64 * r)];
 }
@@ -137,10 +156,10 @@ __device__ __forceinline__ void wave_split_pair(const C2<T> (&v)[WaveShape<N>::E
                                                 T (&ar)[WaveShape<N>::E / 2 + 1], T (&ai)[WaveShape<N>::E / 2 + 1],
                                                 T (&br)[WaveShape<N>::E / 2 + 1], T (&bi)[WaveShape<N>::E / 2 + 1]) {
     constexpr int E = WaveShape<N>::E, NT = E / 2 + 1;
-    __syncthreads();
+    wave_lds_sync();
 #pragma unroll
     for (int r = 0; r < E; r++) buf[wpad(lane + 64 * r)] = v[r];
-    __syncthreads();
+    wave_lds_sync();
     const T sc = T(0.5) / sqrt(T(N));
 #pragma unroll
     for (int t = 0; t < NT; t++) {
@@ -270,7 +289,8 @@ __global__ void __launch_bounds__(64 * NW) fftw_inv_pass(const C2<T>* __restrict
 
 // Inverse pass, pointwise combination and forward pass along the same axis in one launch.  For slab
 // group s (block-uniform) the block's rows r are the inverse transforms of K = comb.count(s) Hermitian
-// inputs, pro(s, k, r, f) = input k, row r, packed position f; each transformed value val (row r,
+// inputs, pro(s, k, r, f) = input k, row r, packed position f (or, when comb.deriv_of_prev(s, k), the
+// derivative of input k - 1 along the transform axis); each transformed value val (row r,
 // position j) is folded into a per-element accumulator, a = a + comb.coef(s, k, r, j) * val, which
 // starts from comb.init(s, r, j); the accumulated real row is transformed forward again and stored
 // transposed: element (r, f) at out[r + n f + n n s].  Nothing between the two transforms goes
@@ -299,8 +319,12 @@ __global__ void __launch_bounds__(64 * NW) fftw_inv_comb_fwd(const C2<T>* __rest
     C2<T> v[E];
 #pragma unroll 1
     for (int k = 0; k < K; k++) {
-        wave_pack_herm<N>(v, buf, lane, h);
-        if (k + 1 < K) wave_fetch_herm<N>(h, lane, [&](int hh, int f) { return pro(s, k + 1, ra + hh, f); });
+        // an input that is the derivative along the transform axis of the previous one is packed from
+        // the previous input's registers, which are kept instead of being refilled with a fetch
+        const bool dcur = k > 0 && comb.deriv_of_prev(s, k), dnext = k + 1 < K && comb.deriv_of_prev(s, k + 1);
+        if (dcur) wave_pack_herm<N, true>(v, buf, lane, h);
+        else wave_pack_herm<N>(v, buf, lane, h);
+        if (k + 1 < K && !dnext) wave_fetch_herm<N>(h, lane, [&](int hh, int f) { return pro(s, k + 1, ra + hh, f); });
         C2<T> cw[E];
 #pragma unroll
         for (int r = 0; r < E; r++) {
