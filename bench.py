@@ -108,10 +108,11 @@ def measured_traffic(site, kernel, ng, nz, precision):
             + float(kv["write_kib_per_dispatch"]) * float(kv["write_scale"])) * kib
 
 
-def kernel_names(ng, precision):
-    t, nw = ("double", 4) if precision == "fp64" else ("float", 8)
-    return {"psolve_deriv": f"fftw_inv_pass<{t}, {ng}, {nw}, true, psolve theta-derivative prologue>",
-            "psolve_comb": f"fftw_inv_comb_fwd<{t}, {ng}, {nw}, psolve source fold>"}
+def kernel_names(ng, nz, precision):
+    t = "double" if precision == "fp64" else "float"
+    return {"psolve_deriv": f"fftw_inv_pass<{t}, {ng}, 8, true, psolve theta-derivative prologue>",
+            "psolve_comb": f"fftw_inv_comb_fwd<{t}, {ng}, 8, psolve source fold>",
+            "psolve_tri": f"fftw_fwd_rows_cols<{t}, {ng}, 8, {nz + 1}, psolve column solve>"}
 
 
 def rel_l2(a, b):
@@ -184,7 +185,7 @@ def main():
     res = {}
     for p, name in ((sw.FP64, "fp64"), (sw.MIXED, "mixed")):
         spec, st = new_state(sw, ng, nz, p, init)
-        modes = {"psolve_deriv": sw.PROF_PSOLVE_DERIV, "psolve_comb": sw.PROF_PSOLVE_COMB}
+        modes = {"psolve_deriv": sw.PROF_PSOLVE_DERIV, "psolve_comb": sw.PROF_PSOLVE_COMB, "psolve_tri": sw.PROF_PSOLVE_TRI}
         ms, iters, prof = time_steps(sw, spec, st, args.steps, args.warmup, modes)
         res[name] = dict(ms=ms, iters=iters, prof=prof, value=points / (ms * 1e-3))
         print(f"[bench] {name}: {ms:.3f} ms/step, {points / (ms * 1e-3):.4g} point-steps/s, psolve iterations "
@@ -201,7 +202,7 @@ def main():
     site = max(h["prof"], key=lambda k: h["prof"][k]["ms_per_step"])
     pk = h["prof"][site]
     achieved = pk["bytes"] / (pk["ms"] * 1e-3) / 1e9 if pk["ms"] > 0 else 0.0
-    names = kernel_names(ng, head)
+    names = kernel_names(ng, nz, head)
     kname = names[site]
     others = {k: {"kernel": names[k], "ms_per_step": v["ms_per_step"], "avg_launch_us": v["ms"] * 1e3 / max(v["launches"], 1),
                   "achieved_GBs": v["bytes"] / (v["ms"] * 1e-3) / 1e9 if v["ms"] > 0 else 0.0}

@@ -94,11 +94,14 @@ int sw_diagnose(sw_state* st, double diag[5]);                        /* diagnos
 int sw_advance(sw_state* st, double diag[5], int iters[3]);           /* advance.rs:13  */
 
 /* ---- instrumentation and controls (no reference counterpart) ---- */
-/* Time one kind of FFT pass launch of this handle with HIP events on the library stream:
+/* Time one kind of launch of this handle with HIP events on the library stream:
  * SW_PROF_STRAIGHT = the straight (second) pass of every 2-D transform, SW_PROF_PSOLVE_DERIV = the
  * transposing pass of psolve's theta-derivative transform, SW_PROF_PSOLVE_COMB = psolve's fused
- * inverse-transform / source / forward-transform launch (ng = 128, 256, 512). */
-typedef enum { SW_PROF_OFF = 0, SW_PROF_STRAIGHT = 1, SW_PROF_PSOLVE_DERIV = 2, SW_PROF_PSOLVE_COMB = 3 } sw_prof_mode;
+ * inverse-transform / source / forward-transform launch, SW_PROF_PSOLVE_TRI = psolve's fused forward
+ * pass and column solve (the last two for ng = 128, 256, 512). */
+typedef enum {
+    SW_PROF_OFF = 0, SW_PROF_STRAIGHT = 1, SW_PROF_PSOLVE_DERIV = 2, SW_PROF_PSOLVE_COMB = 3, SW_PROF_PSOLVE_TRI = 4
+} sw_prof_mode;
 int sw_profile(sw_spectral* s, int mode);
 int sw_profile_read(sw_spectral* s, double* ms, double* bytes, long* launches);
 /* psolve stopping tolerance; default 1e-9 (psolve.rs:17) in FP64, 1e-6 in the mixed mode, where the

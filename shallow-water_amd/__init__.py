@@ -20,7 +20,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libsw_hip.so")
 
 FP64, MIXED = 0, 1
-PROF_OFF, PROF_STRAIGHT, PROF_PSOLVE_DERIV, PROF_PSOLVE_COMB = 0, 1, 2, 3
+PROF_OFF, PROF_STRAIGHT, PROF_PSOLVE_DERIV, PROF_PSOLVE_COMB, PROF_PSOLVE_TRI = 0, 1, 2, 3, 4
 FIELDS = ["u", "v", "w", "z", "zx", "zy", "r", "ri", "aa", "zeta", "pn", "dpn", "ps", "qs", "ds", "gs"]
 STATUS = {0: "SW_OK", -1: "SW_ERR_ARG", -2: "SW_ERR_PSOLVE_MAXITER", -3: "SW_ERR_PSOLVE_DIVERGED",
           -4: "SW_ERR_FACTOR", -5: "SW_ERR_UNSUPPORTED", -6: "SW_ERR_HIP", -7: "SW_ERR_NONFINITE"}
@@ -154,8 +154,8 @@ class Spectral:
     def profile(self, mode=PROF_STRAIGHT):
         """Time one kind of FFT pass launch of this handle with HIP events (PROF_STRAIGHT: the straight
         pass of every 2-D transform; PROF_PSOLVE_DERIV: the transposing pass of psolve's theta-derivative
-        transform; PROF_PSOLVE_COMB: psolve's fused inverse-transform / source / forward-transform launch);
-        PROF_OFF stops."""
+        transform; PROF_PSOLVE_COMB: psolve's fused inverse-transform / source / forward-transform launch;
+        PROF_PSOLVE_TRI: psolve's fused forward pass and column solve); PROF_OFF stops."""
         _check(_lib.sw_profile(self._h, C.c_int(int(mode))))
 
     def profile_read(self):

@@ -580,7 +580,7 @@ int sw_advance(sw_state* st, double diag[5], int iters[3]) {
 
 int sw_profile(sw_spectral* s, int mode) {
     return guard([&] {
-        SW_CHECK(mode >= SW_PROF_OFF && mode <= SW_PROF_PSOLVE_COMB, SW_ERR_ARG, "unknown profiling mode");
+        SW_CHECK(mode >= SW_PROF_OFF && mode <= SW_PROF_PSOLVE_TRI, SW_ERR_ARG, "unknown profiling mode");
         return with_engine(s, [&](auto& e) {
             e.prof_collect();
             e.prof = mode;

@@ -154,6 +154,51 @@ struct PsolveComb {
     }
 };
 
+// psolve's column solve (psolve.rs:285-346) for one (kx, ky): the fourth-order compact tridiagonal in
+// theta with the precomputed LU (htdv, etdv, ap), in FP64, on the column c[iz n] = S(kx, ky, iz) of the
+// source spectrum; writes ps and the top level of S (used after the iteration), and returns the
+// Parseval-weighted squared change and squared old value of ps for the residual.  NZC = nz.
+template <typename T, int NZC>
+struct PsolveCol {
+    const double *htdv, *etdv, *ap;
+    T* ps;
+    T* sptop;
+    int n;
+    size_t nn;
+    __device__ __forceinline__ Acc<2> operator()(int kx, int ky, const T* c) const {
+        const size_t p = kx + (size_t)n * ky;
+        const double wgt = ((kx == 0 || 2 * kx == n) ? 1.0 : 2.0) * ((ky == 0 || 2 * ky == n) ? 1.0 : 2.0);
+        auto S = [&](int iz) { return (double)c[iz * n]; };
+        double colr[NZC];
+        double prev = ((1.0 / 3.0) * S(0) + (1.0 / 6.0) * S(1)) * htdv[p];
+        const double a = ap[p];
+        colr[0] = prev;
+#pragma unroll
+        for (int iz = 1; iz < NZC; iz++) {
+            const double gg = (1.0 / 12.0) * (S(iz - 1) + S(iz + 1)) + (5.0 / 6.0) * S(iz);
+            prev = (gg - a * prev) * htdv[(size_t)iz * nn + p];
+            colr[iz] = prev;
+        }
+        double nxt = prev, dsum = 0.0, osum = 0.0;
+#pragma unroll
+        for (int iz = NZC - 1; iz >= 0; iz--) {
+            const size_t o = (size_t)iz * nn + p;
+            if (iz < NZC - 1) nxt = colr[iz] + etdv[o] * nxt;
+            const T neu = T(nxt);
+            const double old = (double)ps[o];
+            const double d = (double)neu - old;
+            dsum += d * d;
+            osum += old * old;
+            ps[o] = neu;
+        }
+        sptop[p] = c[NZC * n];
+        Acc<2> r;
+        r.v[0] = wgt * dsum;
+        r.v[1] = wgt * osum;
+        return r;
+    }
+};
+
 template <typename T>
 struct Engine {
     int ng = 0, nz = 0, L = 0;
@@ -300,7 +345,7 @@ struct Engine {
     }
 
     // call sites a profiling mode can single out
-    enum { SITE_OTHER = 0, SITE_PSOLVE_DERIV = 1, SITE_PSOLVE_COMB = 2 };
+    enum { SITE_OTHER = 0, SITE_PSOLVE_DERIV = 1, SITE_PSOLVE_COMB = 2, SITE_PSOLVE_TRI = 3 };
 
     struct NoEpi {
         __device__ void operator()(int, int, int, T) const {}
@@ -399,7 +444,8 @@ struct Engine {
     static constexpr int NWT = 8;
     bool timed(int site) const {
         return (prof == SW_PROF_PSOLVE_DERIV && site == SITE_PSOLVE_DERIV) ||
-               (prof == SW_PROF_PSOLVE_COMB && site == SITE_PSOLVE_COMB);
+               (prof == SW_PROF_PSOLVE_COMB && site == SITE_PSOLVE_COMB) ||
+               (prof == SW_PROF_PSOLVE_TRI && site == SITE_PSOLVE_TRI);
     }
     template <int NN, int NW, class Pro>
     void fwd_pass1_wave(int nslab, Pro pro, T* out, int site) {

@@ -312,8 +312,8 @@ __global__ void __launch_bounds__(64 * NW) fftw_inv_comb_fwd(const C2<T>* __rest
     }
     const int K = comb.count(s);
     const T sc = T(1) / sqrt(T(N));
-    // software pipeline: input k + 1 and the coefficients of input k are in flight during the
-    // transform of input k
+    // software pipeline: input k + 1 is in flight during the transform of input k (the coefficients
+    // are loaded after it: prefetching them too costs more in registers than it gains)
     HermPair<N, T> h;
     wave_fetch_herm<N>(h, lane, [&](int hh, int f) { return pro(s, 0, ra + hh, f); });
     C2<T> v[E];
@@ -325,23 +325,99 @@ __global__ void __launch_bounds__(64 * NW) fftw_inv_comb_fwd(const C2<T>* __rest
         if (dcur) wave_pack_herm<N, true>(v, buf, lane, h);
         else wave_pack_herm<N>(v, buf, lane, h);
         if (k + 1 < K && !dnext) wave_fetch_herm<N>(h, lane, [&](int hh, int f) { return pro(s, k + 1, ra + hh, f); });
-        C2<T> cw[E];
-#pragma unroll
-        for (int r = 0; r < E; r++) {
-            const int j = lane + 64 * r;
-            cw[r] = C2<T>{comb.coef(s, k, ra, j), comb.coef(s, k, ra + 1, j)};
-        }
         wave_stockham<N, true>(v, buf, tw, lane);
 #pragma unroll
         for (int r = 0; r < E; r++) {
-            acc[r].x = acc[r].x + cw[r].x * (v[r].x * sc);
-            acc[r].y = acc[r].y + cw[r].y * (v[r].y * sc);
+            const int j = lane + 64 * r;
+            acc[r].x = acc[r].x + comb.coef(s, k, ra, j) * (v[r].x * sc);
+            acc[r].y = acc[r].y + comb.coef(s, k, ra + 1, j) * (v[r].y * sc);
         }
     }
     wave_stockham<N, false>(acc, buf, tw, lane);
     T ar[NT], ai[NT], br[NT], bi[NT];
     wave_split_pair<N>(acc, buf, lane, ar, ai, br, bi);
     wave_store_trans<N, NW>(ar, ai, br, bi, reinterpret_cast<T*>(lds_raw), lane, wv, tout + (size_t)s * N * N + r0);
+}
+
+// Forward pass along x of every level of one row, then a per-column functor across the levels.  Block
+// ky owns row ky of the LV slabs ld(l, ky, x) (l < LV); its NWV waves transform the rows as level pairs
+// (pair m on wave m mod NWV), the spectra land in an LDS column store cs[l N + kx], and each thread then
+// calls col(kx, ky, cs + kx) for its columns (level l at offset l N).  The Acc<2> results are summed
+// over the block, in a fixed order, into part[2 ky + c].
+template <typename T, int N, int NWV, int LV, class Ld, class Col>
+__global__ void __launch_bounds__(64 * NWV) fftw_fwd_rows_cols(const C2<T>* __restrict__ tw, Ld ld, Col col,
+                                                              double* part) {
+    constexpr int E = WaveShape<N>::E, NT = E / 2 + 1, NP = (LV + 1) / 2, MAXP = (NP + NWV - 1) / NWV;
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    C2<T>* buf = reinterpret_cast<C2<T>*>(lds_raw) + wv * wave_region<N>();
+    T* cs = reinterpret_cast<T*>(lds_raw);
+    const int ky = blockIdx.x;
+
+    T ar[MAXP][NT], ai[MAXP][NT], br[MAXP][NT], bi[MAXP][NT];
+#pragma unroll
+    for (int m = 0; m < MAXP; m++) {
+        const int pr = wv + m * NWV;  // wave-uniform
+        if (pr < NP) {
+            const int l0 = 2 * pr, l1 = 2 * pr + 1;
+            C2<T> v[E];
+#pragma unroll
+            for (int r = 0; r < E; r++) {
+                const int x = lane + 64 * r;
+                v[r] = C2<T>{ld(l0, ky, x), l1 < LV ? ld(l1, ky, x) : T(0)};
+            }
+            wave_stockham<N, false>(v, buf, tw, lane);
+            wave_split_pair<N>(v, buf, lane, ar[m], ai[m], br[m], bi[m]);
+        }
+    }
+    __syncthreads();  // every wave is done with its exchange region (the column store overlays them)
+#pragma unroll
+    for (int m = 0; m < MAXP; m++) {
+        const int pr = wv + m * NWV;
+        if (pr < NP) {
+            const int l0 = 2 * pr, l1 = 2 * pr + 1;
+#pragma unroll
+            for (int t = 0; t < NT; t++) {
+                const int k = lane + 64 * t;
+                if (t < NT - 1 || lane == 0) {
+                    cs[l0 * N + k] = ar[m][t];
+                    if (l1 < LV) cs[l1 * N + k] = br[m][t];
+                    if (k > 0 && 2 * k < N) {
+                        cs[l0 * N + N - k] = ai[m][t];
+                        if (l1 < LV) cs[l1 * N + N - k] = bi[m][t];
+                    }
+                }
+            }
+        }
+    }
+    __syncthreads();
+    double s0 = 0.0, s1 = 0.0;
+    for (int kx = threadIdx.x; kx < N; kx += 64 * NWV) {
+        const auto a = col(kx, ky, cs + kx);
+        s0 += a.v[0];
+        s1 += a.v[1];
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        s0 += __shfl_down(s0, off, 64);
+        s1 += __shfl_down(s1, off, 64);
+    }
+    __syncthreads();  // the column store is no longer read
+    double* red = reinterpret_cast<double*>(lds_raw);
+    if (lane == 0) red[2 * wv] = s0, red[2 * wv + 1] = s1;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double a0 = red[0], a1 = red[1];
+        for (int w = 1; w < NWV; w++) a0 += red[2 * w], a1 += red[2 * w + 1];
+        part[2 * ky] = a0;
+        part[2 * ky + 1] = a1;
+    }
+}
+
+template <int N, int NWV, int LV, typename T>
+constexpr size_t rows_cols_lds_bytes() {
+    size_t ex = (size_t)NWV * wave_region<N>() * sizeof(C2<T>);
+    size_t cs = (size_t)LV * N * sizeof(T);
+    return ex > cs ? ex : cs;
 }
 
 }  // namespace sw
