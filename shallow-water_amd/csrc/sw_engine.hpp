@@ -468,11 +468,13 @@ struct Engine {
         if (tm) prof_stop();
         SW_HIP(hipGetLastError());
     }
+    // bytes: algorithmic bytes of the launch for the profiling mode that times this site (0: read +
+    // write of every transformed value)
     template <int NN, int NW, class Pro>
-    void inv_pass1_wave(int nslab, Pro pro, T* out, int site) {
+    void inv_pass1_wave(int nslab, Pro pro, T* out, int site, double bytes = 0.0) {
         const size_t lds_t = wave_lds_bytes<NN, NW, T>();
         const bool tm = timed(site);
-        if (tm) prof_start(2.0 * sizeof(T) * (double)n2 * nslab);
+        if (tm) prof_start(bytes > 0.0 ? bytes : 2.0 * sizeof(T) * (double)n2 * nslab);
         hipLaunchKernelGGL((fftw_inv_pass<T, NN, NW, true, Pro, NoEpi>), dim3(nslab * (NN / (2 * NW))), dim3(64 * NW),
                            lds_t, st, plan.tw, nslab, pro, NoEpi{}, out);
         if (tm) prof_stop();
