@@ -121,3 +121,19 @@ def test_one_step_synthetic(sw, oracle, ng, nz):
     assert rc == 0 and iters == iters2
     for f in ALL:
         assert rel_l2(g.get(f), getattr(o, f)) < STEP_TOL[f], f
+
+
+def test_one_step_synthetic_large_mixed_radix(sw, oracle):
+    # ng = 1536 = 2^9 3: the block-per-rows transform with a radix-3 stage (the reference factorises it
+    # as 6 4^4); one step against the oracle.  The bounds are those of the 512 and smaller cases times
+    # ng/512: a transform's rounding floor is white in wavenumber, and the spectral derivatives inside
+    # the step amplify it by up to the largest retained wavenumber (ng/3), three times larger here.
+    ng, nz = 1536, 4
+    g, o = synthetic(sw, oracle, ng, nz, seed=2)
+    diag, iters = sw.advance(g)
+    rc, diag2, iters2 = oracle.advance(o)
+    assert rc == 0 and iters == iters2
+    errs = {f: rel_l2(g.get(f), getattr(o, f)) for f in ALL}
+    print("1536x4 one step, rel L2 vs oracle:", {f: f"{e:.1e}" for f, e in errs.items()})
+    for f in ALL:
+        assert errs[f] < STEP_TOL[f] * ng / 512, f
