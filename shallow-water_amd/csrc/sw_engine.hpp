@@ -11,6 +11,7 @@
 #include <map>
 #include <string>
 #include <type_traits>
+#include <climits>
 
 #include "sw_common.hpp"
 #include "sw_fft.hpp"
@@ -199,6 +200,80 @@ struct PsolveCol {
     }
 };
 
+// Convergence control of psolve's iteration when it runs without a host round trip per iteration:
+// the state of the host loop psolve.rs:339-366 (as restated on the host in the block-per-rows path)
+// lives in device memory and k_psolve_finalize advances it after each iteration's column solve.
+struct PsolveCtrl {
+    int done;    // no further iteration: every gated launch of later iterations is a no-op
+    int iters;   // iterations completed
+    int status;  // 0, SW_ERR_NONFINITE or SW_ERR_PSOLVE_DIVERGED
+    int stall;   // mixed precision: consecutive iterations whose residual failed to halve
+    double errp, errp_prev;
+    double top2;  // squared norm of the caller's top level of pn (enters the first residual)
+};
+
+// Resets the control state; top2 = the nb block partials of its reduction summed in block order
+// (Kahan-compensated), as the host does for the other reductions.
+__global__ void k_psolve_ctrl_init(PsolveCtrl* c, const double* top2_part, int nb) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    double acc = 0.0, cmp = 0.0;
+    for (int b = 0; b < nb; b++) {
+        double y = top2_part[b] - cmp, t = acc + y;
+        cmp = (t - acc) - y;
+        acc = t;
+    }
+    *c = PsolveCtrl{0, 0, 0, 0, 1.0, 1.0, acc};
+}
+
+// One block of 256 threads: the residual sums from the nb block partials of the column solve (Kahan per
+// thread over a fixed set of blocks, then a fixed shuffle tree), then the stopping rule: errp =
+// sqrt(dsum / (osum + 1e-20)) with the top level of the caller's pn added at the first iteration
+// (c->top2); non-finite or (after the first iteration) errp > 1 stop with an error; errp <= toler or
+// nitmax iterations stop; in the mixed mode also two consecutive iterations from the third on whose
+// residual failed to halve while below 1e-4.
+__global__ void __launch_bounds__(256) k_psolve_finalize(const double* part, int nb, double toler, int nitmax,
+                                                         int mixed, PsolveCtrl* c) {
+    __shared__ double sh[4][2];
+    if (c->done) return;
+    double acc[2] = {0.0, 0.0}, cmp[2] = {0.0, 0.0};
+    for (int b = threadIdx.x; b < nb; b += blockDim.x)
+        for (int k = 0; k < 2; k++) {
+            double y = part[2 * b + k] - cmp[k], t = acc[k] + y;
+            cmp[k] = (t - acc[k]) - y;
+            acc[k] = t;
+        }
+    for (int k = 0; k < 2; k++) {
+        double x = acc[k];
+        for (int off = 32; off > 0; off >>= 1) x += __shfl_down(x, off, 64);
+        if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6][k] = x;
+    }
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    double s0 = sh[0][0] + sh[1][0] + sh[2][0] + sh[3][0];
+    double s1 = sh[0][1] + sh[1][1] + sh[2][1] + sh[3][1];
+    if (c->iters == 0) s0 += c->top2, s1 += c->top2;
+    const double errp = sqrt(s0 / (s1 + 1.0E-20));
+    c->errp = errp;
+    if (!isfinite(errp)) {
+        c->iters += 1, c->status = SW_ERR_NONFINITE, c->done = 1;
+        return;
+    }
+    if (c->iters > 0 && errp > 1.0) {
+        c->iters += 1, c->status = SW_ERR_PSOLVE_DIVERGED, c->done = 1;
+        return;
+    }
+    c->iters += 1;
+    if (mixed && c->iters >= 3 && errp > toler) {
+        c->stall = errp > 0.5 * c->errp_prev ? c->stall + 1 : 0;
+        if (c->stall >= 2 && errp < 1.0e-4) {
+            c->done = 1;
+            return;
+        }
+    }
+    c->errp_prev = errp;
+    if (errp <= toler || c->iters >= nitmax) c->done = 1;
+}
+
 template <typename T>
 struct Engine {
     int ng = 0, nz = 0, L = 0;
@@ -224,6 +299,10 @@ struct Engine {
     size_t ev_used = 0;
     double prof_ms = 0.0, prof_bytes = 0.0;
     long prof_launches = 0;
+    // tag: the psolve iteration a launch belongs to in a device-controlled loop (-1: none); launches
+    // of iterations at or beyond the loop's final count were no-ops and are not counted
+    std::vector<int> ev_tag;
+    int prof_tag = -1;
     void prof_start(double bytes) {
         if (ev_used == ev.size()) {
             hipEvent_t a, b;
@@ -231,18 +310,21 @@ struct Engine {
             SW_HIP(hipEventCreate(&b));
             ev.push_back({a, b});
             ev_bytes.push_back(0.0);
+            ev_tag.push_back(-1);
         }
         ev_bytes[ev_used] = bytes;
+        ev_tag[ev_used] = prof_tag;
         SW_HIP(hipEventRecord(ev[ev_used].first, st));
     }
     void prof_stop() {
         SW_HIP(hipEventRecord(ev[ev_used].second, st));
         ev_used++;
     }
-    void prof_collect() {
+    void prof_collect(int drop_from = INT_MAX) {
         if (!ev_used) return;
         SW_HIP(hipStreamSynchronize(st));
         for (size_t i = 0; i < ev_used; i++) {
+            if (ev_tag[i] >= drop_from) continue;
             float ms = 0.f;
             SW_HIP(hipEventElapsedTime(&ms, ev[i].first, ev[i].second));
             prof_ms += ms;
@@ -257,6 +339,15 @@ struct Engine {
             (void)hipEventDestroy(e.second);
         }
     }
+    // device flag that turns the gated launches into no-ops (set while a device-controlled psolve
+    // loop is being enqueued; null otherwise)
+    const int* gate_ = nullptr;
+    DevBuf<PsolveCtrl> ctrl;
+    // first batch of a device-controlled psolve loop: the iteration count of the previous call in the
+    // same position (advance's predictor and two correctors; slot 3 for any other call), which
+    // changes slowly from step to step
+    int iter_hint[4] = {8, 8, 8, 8};
+    int psolve_slot = 3;
     // psolve controls
     double toler = 1.0e-9;
     int last_iters = 0;
@@ -363,6 +454,7 @@ struct Engine {
 
     // f(integral_constant<int, nz>) for the level counts with a compile-time specialisation, else
     // f(integral_constant<int, 0>).
+    bool nz_spec() const { return nz == 4 || nz == 8 || nz == 16 || nz == 24 || nz == 32; }
     template <class F>
     void nz_dispatch(F f) {
         switch (nz) {
@@ -476,7 +568,7 @@ struct Engine {
         const bool tm = timed(site);
         if (tm) prof_start(bytes > 0.0 ? bytes : 2.0 * sizeof(T) * (double)n2 * nslab);
         hipLaunchKernelGGL((fftw_inv_pass<T, NN, NW, true, Pro, NoEpi>), dim3(nslab * (NN / (2 * NW))), dim3(64 * NW),
-                           lds_t, st, plan.tw, nslab, pro, NoEpi{}, out);
+                           lds_t, st, plan.tw, nslab, pro, NoEpi{}, out, gate_);
         if (tm) prof_stop();
         SW_HIP(hipGetLastError());
     }
@@ -487,7 +579,7 @@ struct Engine {
         if (tm) prof_start(2.0 * sizeof(T) * (double)n2 * nslab);
         LoadPlain<T> lp{in, n2, ng};
         hipLaunchKernelGGL((fftw_inv_pass<T, NN, 1, false, LoadPlain<T>, Epi>), dim3(nslab * (NN / 2)), dim3(64),
-                           lds_s, st, plan.tw, nslab, lp, epi, (T*)nullptr);
+                           lds_s, st, plan.tw, nslab, lp, epi, (T*)nullptr, gate_);
         if (tm) prof_stop();
         SW_HIP(hipGetLastError());
     }
@@ -499,7 +591,7 @@ struct Engine {
         const bool tm = timed(site);
         if (tm) prof_start(bytes);
         hipLaunchKernelGGL((fftw_inv_comb_fwd<T, NN, NWT, Pro, Comb>), dim3(nslab * (NN / (2 * NWT))), dim3(64 * NWT),
-                           lds_t, st, plan.tw, nslab, pro, comb, out);
+                           lds_t, st, plan.tw, nslab, pro, comb, out, gate_);
         if (tm) prof_stop();
         SW_HIP(hipGetLastError());
     }

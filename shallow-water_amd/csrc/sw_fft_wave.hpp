@@ -247,7 +247,8 @@ __global__ void __launch_bounds__(64 * NW) fftw_fwd_pass(const C2<T>* __restrict
 // TRANS: element (r, x) goes to out[r + n x + n n s]; otherwise epi(s, r, x, value).
 template <typename T, int N, int NW, bool TRANS, class Pro, class Epi>
 __global__ void __launch_bounds__(64 * NW) fftw_inv_pass(const C2<T>* __restrict__ tw, int nslab, Pro pro, Epi epi,
-                                                        T* tout) {
+                                                        T* tout, const int* gate) {
+    if (gate && *gate) return;  // a finished device-controlled iteration (see k_psolve_finalize)
     constexpr int E = WaveShape<N>::E, BPS = N / (2 * NW);
     extern __shared__ __align__(16) unsigned char lds_raw[];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
@@ -296,7 +297,9 @@ __global__ void __launch_bounds__(64 * NW) fftw_inv_pass(const C2<T>* __restrict
 // transposed: element (r, f) at out[r + n f + n n s].  Nothing between the two transforms goes
 // through HBM.
 template <typename T, int N, int NW, class Pro, class Comb>
-__global__ void __launch_bounds__(64 * NW) fftw_inv_comb_fwd(const C2<T>* __restrict__ tw, int nslab, Pro pro, Comb comb, T* tout) {
+__global__ void __launch_bounds__(64 * NW) fftw_inv_comb_fwd(const C2<T>* __restrict__ tw, int nslab, Pro pro, Comb comb, T* tout,
+                                                            const int* gate) {
+    if (gate && *gate) return;
     constexpr int E = WaveShape<N>::E, BPS = N / (2 * NW), NT = E / 2 + 1;
     extern __shared__ __align__(16) unsigned char lds_raw[];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
@@ -343,10 +346,12 @@ __global__ void __launch_bounds__(64 * NW) fftw_inv_comb_fwd(const C2<T>* __rest
 // ky owns row ky of the LV slabs ld(l, ky, x) (l < LV); its NWV waves transform the rows as level pairs
 // (pair m on wave m mod NWV), the spectra land in an LDS column store cs[l N + kx], and each thread then
 // calls col(kx, ky, cs + kx) for its columns (level l at offset l N).  The Acc<2> results are summed
-// over the block, in a fixed order, into part[2 ky + c].
+// over the block, in a fixed order, into part[2 ky + c].  A non-null gate that reads non-zero makes
+// the launch a no-op (fftw_inv_pass and fftw_inv_comb_fwd take the same argument).
 template <typename T, int N, int NWV, int LV, class Ld, class Col>
 __global__ void __launch_bounds__(64 * NWV) fftw_fwd_rows_cols(const C2<T>* __restrict__ tw, Ld ld, Col col,
-                                                              double* part) {
+                                                              double* part, const int* gate) {
+    if (gate && *gate) return;
     constexpr int E = WaveShape<N>::E, NT = E / 2 + 1, NP = (LV + 1) / 2, MAXP = (NP + NWV - 1) / NWV;
     extern __shared__ __align__(16) unsigned char lds_raw[];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;

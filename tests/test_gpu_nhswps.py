@@ -89,6 +89,14 @@ def test_psolve_iteration_cap(sw):
     assert e.value.code == -2
 
 
+def test_psolve_iteration_cap_device_loop(sw, oracle):
+    # ng = 128, nz = 4 runs psolve's iteration under device-side control: the cap must still stop it
+    g, _ = synthetic(sw, oracle, 128, 4, seed=3)
+    with pytest.raises(sw.SwError) as e:
+        sw.psolve(g, nitmax=2)
+    assert e.value.code == -2
+
+
 @pytest.mark.parametrize("ng,nz,t0", [(18, 6, 5.555_555_555_555_555E-2), (24, 4, 0.624_999_999_999_999_9)])
 def test_advance_golden(sw, oracle, ng, nz, t0):
     # src/nhswps/advance.rs:296-600: 16 fields and t at abs 1e-10
