@@ -186,6 +186,50 @@ def long_run_errors(sw, ng, nz, init, steps):
             "fp64_monitor_t_zrms2half_zrms_zmax_umax": diag}
 
 
+def ensemble_bench(sw, steps, warmup, nmember=64, nstreams=(1, 4)):
+    """The batched-ensemble workload: nmember members, member m with ng = (64, 128, 256)[m % 3],
+    nz = (4, 8)[(m // 3) % 2], k0 = (4, 8, 16)[(m // 6) % 3], seed 1000 + m, seeded k^-3 PV with rms
+    zeta = 0.3 f (at f/2 the reference algorithm's pressure iteration diverges for the k0 = 4, ng = 64
+    members, and the FP32 bulk loses some k0 = 16 members), mixed precision (FP32 bulk, FP64 psolve
+    tridiagonal and residual), each member with its own spectral handle, advanced together by
+    sw_ensemble_advance over 1 and 4 streams.  Returns the aggregate point-steps/s per stream count."""
+    shapes = [((64, 128, 256)[m % 3], (4, 8)[(m // 3) % 2], (4.0, 8.0, 16.0)[(m // 6) % 3], 1000 + m)
+              for m in range(nmember)]
+    points = sum(ng * ng * (nz + 1) for ng, nz, _, _ in shapes)
+    out = {"members": nmember, "points_per_step": points, "precision": "mixed", "rms_zeta": "0.3 f",
+           "shapes": "ng=(64,128,256)[m%3], nz=(4,8)[(m//3)%2], k0=(4,8,16)[(m//6)%3], seed=1000+m"}
+    for ns in nstreams:
+        states = []
+        for ng, nz, k0, seed in shapes:
+            qs, ds = sw.synth_kminus3(ng, nz, seed=seed, k0=k0, zeta_rms=0.3 * 4 * np.pi)
+            gs = np.zeros_like(qs)
+            s64 = sw.Spectral(ng, nz, sw.FP64)
+            r, u, v, zeta = s64.field(), s64.field(), s64.field(), s64.field()
+            s64.main_invert(qs, ds, gs, r, u, v, zeta)
+            del s64
+            states.append(sw.State(sw.Spectral(ng, nz, sw.MIXED), ggen=True, qs=qs, ds=ds, gs=gs, r=r, u=u, v=v,
+                                   zeta=zeta))
+        ens = sw.Ensemble(states, nstreams=ns)
+
+        def step():
+            for st in states:
+                st.set_time(st.t, True)
+            ens.advance()
+
+        for _ in range(warmup):
+            step()
+        sw.sync()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            step()
+        dt = (time.perf_counter() - t0) / steps
+        out[f"streams_{ns}"] = {"ms_per_step": dt * 1e3, "point_steps_per_s": points / dt}
+        print(f"[bench] ensemble of {nmember} on {ns} stream(s): {dt * 1e3:.2f} ms/step, {points / dt:.4g} point-steps/s",
+              file=sys.stderr, flush=True)
+        del ens, states
+    return out
+
+
 def cpu_baseline(sw, ng, nz, init):
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import _oracle
@@ -210,6 +254,8 @@ def main():
     ap.add_argument("--nz", type=int, default=16)
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--ensemble-steps", type=int, default=3,
+                    help="timed steps of the 64-member ensemble workload reported beside the headline (0: skip)")
     ap.add_argument("--long-steps", type=int, default=1000,
                     help="steps of the mixed-vs-FP64 accuracy run reported beside the one-step errors (0: skip)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -257,6 +303,8 @@ def main():
         for k, v in long_runs.items():
             print(f"[bench] long run {k}: {v}", file=sys.stderr, flush=True)
 
+    ens = ensemble_bench(sw, args.ensemble_steps, 1) if args.ensemble_steps > 0 else None
+
     # headline: the fastest precision mode that meets the accuracy bar against the FP64 step
     meets = all(v <= ACCURACY_BAR for v in err.values())
     head = "mixed" if meets else "fp64"
@@ -303,6 +351,7 @@ def main():
                      "launches": pk["launches"], "avg_launch_us": pk["ms"] * 1e3 / max(pk["launches"], 1),
                      "algorithmic_bytes_per_launch": pk["bytes"] / max(pk["launches"], 1),
                      "profiled_kernels": others},
+        "ensemble": ens,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

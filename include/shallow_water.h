@@ -93,6 +93,19 @@ int sw_source(sw_state* st, double* sqs, double* sds, double* sgs);   /* source.
 int sw_diagnose(sw_state* st, double diag[5]);                        /* diagnose mod.rs:666 */
 int sw_advance(sw_state* st, double diag[5], int iters[3]);           /* advance.rs:13  */
 
+/* ---- ensemble: members advanced together (no reference function; the ensemble configuration of
+ * the batched workload) ---- */
+typedef struct sw_ensemble sw_ensemble;
+/* members: nmember states, each with its own spectral handle (any ng, nz >= 3, precision); the
+ * ensemble does not own them.  nstreams: HIP streams the members are spread over, each driven by one
+ * host thread (0: 4). */
+int sw_ensemble_create(int nmember, sw_state* const* members, int nstreams, sw_ensemble** out);
+int sw_ensemble_destroy(sw_ensemble* e);
+/* One advance step (advance.rs:13) of every member, as sw_advance does for one state: diag[5 i ..],
+ * iters[3 i ..] and status[i] (0 or a negative sw_status) per member i; returns the first member's
+ * non-zero status, if any. */
+int sw_ensemble_advance(sw_ensemble* e, double* diag, int* iters, int* status);
+
 /* ---- instrumentation and controls (no reference counterpart) ---- */
 /* Time one kind of launch of this handle with HIP events on the library stream:
  * SW_PROF_STRAIGHT = the straight (second) pass of every 2-D transform, SW_PROF_PSOLVE_DERIV = the

@@ -268,6 +268,34 @@ def advance(st):
     return diag, [int(i) for i in iters]
 
 
+class Ensemble:
+    """Member states advanced together (sw_ensemble_*): each member needs its own Spectral handle; the
+    members are spread over `nstreams` HIP streams, each driven by one host thread."""
+
+    def __init__(self, states, nstreams=4):
+        self.states = list(states)
+        arr = (C.c_void_p * len(self.states))(*[st._h.value for st in self.states])
+        h = _vp()
+        _check(_lib.sw_ensemble_create(C.c_int(len(self.states)), arr, C.c_int(nstreams), C.byref(h)))
+        self._h = h
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            _lib.sw_ensemble_destroy(self._h)
+            self._h = None
+
+    def advance(self):
+        """One timestep of every member; returns (monitor values (n, 5), psolve iterations (n, 3),
+        status (n,)); raises SwError when a member failed (its status is the error code)."""
+        n = len(self.states)
+        diag = np.zeros((n, 5))
+        iters = np.zeros((n, 3), np.int32)
+        status = np.zeros(n, np.int32)
+        _check(_lib.sw_ensemble_advance(self._h, _d(diag), iters.ctypes.data_as(C.POINTER(C.c_int)),
+                                        status.ctypes.data_as(C.POINTER(C.c_int))))
+        return diag, iters, status
+
+
 def synth_kminus3(ng, nz, seed, k0=8.0, zeta_rms=0.5 * 4 * np.pi, delta_rms=0.0):
     """Seeded spectral PV (and divergence) with an isotropic k^-3 spectrum peaked at k0, identical on
     every level; returns (qs, ds) as spectral fields."""

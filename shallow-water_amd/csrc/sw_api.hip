@@ -9,6 +9,8 @@
 #include <memory>
 #include <mutex>
 #include <string>
+#include <thread>
+#include <vector>
 
 #include "shallow_water.h"
 #include "sw_engine.hpp"
@@ -121,6 +123,17 @@ struct sw_state {
             else f.a[i] = f32[i].p;
         }
         return f;
+    }
+};
+
+// An ensemble: member states (each with its own spectral handle, hence its own engine and work buffers)
+// advanced together, spread over a few HIP streams each driven by one host thread.
+struct sw_ensemble {
+    std::vector<sw_state*> m;
+    std::vector<hipStream_t> streams;
+    std::vector<std::vector<int>> groups;  // member indices per stream
+    ~sw_ensemble() {
+        for (auto s : streams) (void)hipStreamDestroy(s);
     }
 };
 
@@ -575,6 +588,109 @@ int sw_advance(sw_state* st, double diag[5], int iters[3]) {
         if (diag) std::memcpy(diag, dg, sizeof(dg));
         if (iters) std::memcpy(iters, it, sizeof(it));
         return rc;
+    });
+}
+
+int sw_ensemble_create(int nmember, sw_state* const* members, int nstreams, sw_ensemble** out) {
+    return guard([&] {
+        SW_CHECK(out && members && nmember > 0, SW_ERR_ARG, "bad ensemble arguments");
+        auto e = std::make_unique<sw_ensemble>();
+        for (int i = 0; i < nmember; i++) {
+            sw_state* st = members[i];
+            SW_CHECK(st && st->s && st->s->nz >= 3, SW_ERR_ARG, "ensemble members need a state with nz >= 3");
+            for (int j = 0; j < i; j++)
+                SW_CHECK(members[j]->s != st->s, SW_ERR_ARG, "ensemble members must not share a spectral handle");
+            e->m.push_back(st);
+        }
+        int ns = nstreams > 0 ? nstreams : 4;
+        ns = std::max(1, std::min(ns, nmember));
+        stream();  // the library device is selected
+        for (int g = 0; g < ns; g++) {
+            hipStream_t h;
+            SW_HIP(hipStreamCreateWithFlags(&h, hipStreamNonBlocking));
+            e->streams.push_back(h);
+        }
+        // largest members first, each to the stream with the least work so far (work ~ points, FP64
+        // counted twice)
+        std::vector<int> order(nmember);
+        std::vector<double> work(nmember), load(ns, 0.0);
+        for (int i = 0; i < nmember; i++) {
+            order[i] = i;
+            const sw_spectral* sp = e->m[i]->s;
+            work[i] = (double)(sp->ng * sp->ng * (sp->nz + 1)) * (sp->precision == SW_FP64 ? 2.0 : 1.0);
+        }
+        std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return work[a] > work[b]; });
+        e->groups.assign(ns, {});
+        for (int i : order) {
+            int g = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+            e->groups[g].push_back(i);
+            load[g] += work[i];
+        }
+        *out = e.release();
+        return 0;
+    });
+}
+
+int sw_ensemble_destroy(sw_ensemble* e) {
+    return guard([&] {
+        if (e) {
+            for (auto s : e->streams) (void)hipStreamSynchronize(s);
+            delete e;
+        }
+        return 0;
+    });
+}
+
+int sw_ensemble_advance(sw_ensemble* e, double* diag, int* iters, int* status) {
+    return guard([&] {
+        SW_CHECK(e, SW_ERR_ARG, "null ensemble");
+        const int n = (int)e->m.size();
+        std::vector<int> rc(n, 0);
+        std::vector<double> dg((size_t)5 * n, 0.0);
+        std::vector<int> it((size_t)3 * n, 0);
+        std::vector<std::string> msg(n);
+        auto run_group = [&](int g) {
+            (void)hipSetDevice(g_device);
+            for (int i : e->groups[g]) {
+                sw_state* st = e->m[i];
+                try {
+                    rc[i] = with_state(st, [&](auto& eng, auto f) {
+                        hipStream_t old = eng.st;
+                        eng.st = e->streams[g];
+                        int r = 0;
+                        try {
+                            r = eng.advance(f, st->t, st->ggen != 0, &dg[(size_t)5 * i], &it[(size_t)3 * i]);
+                            SW_HIP(hipStreamSynchronize(eng.st));
+                        } catch (...) {
+                            (void)hipStreamSynchronize(eng.st);
+                            eng.st = old;
+                            throw;
+                        }
+                        eng.st = old;
+                        return r;
+                    });
+                } catch (const Error& x) {
+                    rc[i] = x.code;
+                    msg[i] = x.what();
+                } catch (const std::exception& x) {
+                    rc[i] = SW_ERR_ARG;
+                    msg[i] = x.what();
+                }
+            }
+        };
+        std::vector<std::thread> th;
+        for (int g = 1; g < (int)e->groups.size(); g++) th.emplace_back(run_group, g);
+        run_group(0);
+        for (auto& t : th) t.join();
+        if (diag) std::memcpy(diag, dg.data(), dg.size() * sizeof(double));
+        if (iters) std::memcpy(iters, it.data(), it.size() * sizeof(int));
+        if (status) std::memcpy(status, rc.data(), rc.size() * sizeof(int));
+        for (int i = 0; i < n; i++)
+            if (rc[i]) {
+                g_err = "ensemble member " + std::to_string(i) + ": " + msg[i];
+                return rc[i];
+            }
+        return 0;
     });
 }
 

@@ -83,41 +83,63 @@ inline void each(hipStream_t st, size_t n, F f) {
     SW_HIP(hipGetLastError());
 }
 
-// Host <-> device copies through a page-locked block owned by the library (hipHostMalloc): the
-// runtime never has to lock the caller's pageable memory for a DMA.  Each chunk completes before
-// the block is reused.
+// Host <-> device copies through page-locked blocks owned by the library (hipHostMalloc): the runtime
+// never has to lock the caller's pageable memory for a DMA.  Each copy leases a block from a pool for
+// its duration, so copies issued by different host threads (on different streams) run concurrently.
+// Each chunk completes before the block is reused.
 struct Pinned {
     static std::mutex& mu() {
         static std::mutex m;
         return m;
     }
-    static void* block() {
-        static void* p = nullptr;
-        if (!p) SW_HIP(hipHostMalloc(&p, chunk(), hipHostMallocDefault));
-        return p;
+    static std::vector<void*>& pool() {
+        static std::vector<void*> v;
+        return v;
     }
     static size_t chunk() { return (size_t)8 << 20; }
+    static void* acquire() {
+        {
+            std::lock_guard<std::mutex> lk(mu());
+            if (!pool().empty()) {
+                void* p = pool().back();
+                pool().pop_back();
+                return p;
+            }
+        }
+        void* p = nullptr;
+        SW_HIP(hipHostMalloc(&p, chunk(), hipHostMallocDefault));
+        return p;
+    }
+    static void release(void* p) {
+        std::lock_guard<std::mutex> lk(mu());
+        pool().push_back(p);
+    }
+};
+struct PinLease {
+    void* p;
+    PinLease() : p(Pinned::acquire()) {}
+    ~PinLease() { Pinned::release(p); }
+    PinLease(const PinLease&) = delete;
+    PinLease& operator=(const PinLease&) = delete;
 };
 
 inline void copy_h2d(void* dst, const void* src, size_t bytes, hipStream_t st) {
-    std::lock_guard<std::mutex> lk(Pinned::mu());
-    void* pin = Pinned::block();
+    PinLease pin;
     for (size_t off = 0; off < bytes; off += Pinned::chunk()) {
         size_t b = std::min(Pinned::chunk(), bytes - off);
-        std::memcpy(pin, (const char*)src + off, b);
-        SW_HIP(hipMemcpyAsync((char*)dst + off, pin, b, hipMemcpyHostToDevice, st));
+        std::memcpy(pin.p, (const char*)src + off, b);
+        SW_HIP(hipMemcpyAsync((char*)dst + off, pin.p, b, hipMemcpyHostToDevice, st));
         SW_HIP(hipStreamSynchronize(st));
     }
 }
 
 inline void copy_d2h(void* dst, const void* src, size_t bytes, hipStream_t st) {
-    std::lock_guard<std::mutex> lk(Pinned::mu());
-    void* pin = Pinned::block();
+    PinLease pin;
     for (size_t off = 0; off < bytes; off += Pinned::chunk()) {
         size_t b = std::min(Pinned::chunk(), bytes - off);
-        SW_HIP(hipMemcpyAsync(pin, (const char*)src + off, b, hipMemcpyDeviceToHost, st));
+        SW_HIP(hipMemcpyAsync(pin.p, (const char*)src + off, b, hipMemcpyDeviceToHost, st));
         SW_HIP(hipStreamSynchronize(st));
-        std::memcpy((char*)dst + off, pin, b);
+        std::memcpy((char*)dst + off, pin.p, b);
     }
 }
 
