@@ -93,6 +93,17 @@ int sw_source(sw_state* st, double* sqs, double* sds, double* sgs);   /* source.
 int sw_diagnose(sw_state* st, double diag[5]);                        /* diagnose mod.rs:666 */
 int sw_advance(sw_state* st, double diag[5], int iters[3]);           /* advance.rs:13  */
 
+/* ---- time loop and diagnostics: src/nhswps/mod.rs ---- */
+/* The energies of savegrid (mod.rs:365-420): out = {t, 0, ekin, ekin, epot, etot}, the ecomp.asc row. */
+int sw_energy(sw_state* st, double out[6]);
+/* The time loop of nhswps (mod.rs:285-345) from the state's prognostic fields (qs, ds, gs) and time,
+ * up to t > tsim, with saves every tgsave: monitor rows (t, zrms^2/2, zrms, zmax, umax; one per step,
+ * the monitor.asc line) into monitor[5 i ..] and ecomp rows (one per save) into ecomp[6 i ..], at most
+ * max_monitor / max_ecomp of them; *nmonitor / *necomp = the rows produced, *iters_total = the psolve
+ * iterations.  File output (savegrid's .r4 fields and spectra) is the caller's. */
+int sw_run(sw_state* st, double tsim, double tgsave, double* monitor, int max_monitor, int* nmonitor, double* ecomp,
+           int max_ecomp, int* necomp, int* iters_total);
+
 /* ---- ensemble: members advanced together (no reference function; the ensemble configuration of
  * the batched workload) ---- */
 typedef struct sw_ensemble sw_ensemble;

@@ -47,6 +47,9 @@ int orc_psolve(void* s, double** fields, int nitmax, int* iters, double* errp);
 int orc_source(void* s, double** fields, double* sqs, double* sds, double* sgs);
 int orc_diagnose(void* s, double** fields, double t, double* out5);
 int orc_advance(void* s, double** fields, double* t, int ggen, int nitmax, double* diag5, int* iters3);
+int orc_energy(void* s, double** fields, double t, double* out6);
+int orc_run(void* s, double** fields, double* t, double tsim, double tgsave, int nitmax, double* mon, int maxmon,
+            int* nmon, double* eco, int maxeco, int* neco, int* iters_total);
 #ifdef __cplusplus
 }
 #endif

@@ -195,4 +195,16 @@ int orc_advance(void* s, double** fields, double* t, int ggen, int nitmax, doubl
     *t = st.t;
     return rc;
 }
+int orc_energy(void* s, double** fields, double t, double* out6) {
+    energy(make_state(s, fields, t, 0), out6);
+    return 0;
+}
+int orc_run(void* s, double** fields, double* t, double tsim, double tgsave, int nitmax, double* mon, int maxmon,
+            int* nmon, double* eco, int maxeco, int* neco, int* iters_total) {
+    if (static_cast<Spectral*>(s)->nz < 3 || !(tgsave > 0.0)) return -1;
+    State st = make_state(s, fields, *t, 0);
+    int rc = run(st, tsim, tgsave, nitmax, mon, maxmon, nmon, eco, maxeco, neco, iters_total);
+    *t = st.t;
+    return rc;
+}
 }

@@ -74,6 +74,9 @@ int psolve(State& st, int nitmax, int* iters, double* errp_out);
 void source(const State& st, double* sqs, double* sds, double* sgs);
 void diagnose(const State& st, double out[5]);
 int advance(State& st, int nitmax, double diag[5], int iters[3]);
+void energy(const State& st, double out[6]);
+int run(State& st, double tsim, double tgsave, int nitmax, double* mon, int maxmon, int* nmon, double* eco, int maxeco,
+        int* neco, int* iters_total);
 
 double unrolled_sum(const double* x, size_t n);
 

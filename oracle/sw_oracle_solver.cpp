@@ -743,4 +743,84 @@ int advance(State& st, int nitmax, double diag[5], int iters[3]) {
     return 0;
 }
 
+// The energies savegrid writes to ecomp.asc (src/nhswps/mod.rs:365-420): out = {t, 0, ekin, ekin, epot,
+// etot}.  ekin = 1/2 sum (1+r)(u^2+v^2[+w^2]) with half weight on the bottom (no w) and top levels,
+// times gvol/(2 HBAR); epot = 1/2 gl^2 csq sum ((1/HBAR) z_top - 1)^2.
+void energy(const State& st, double out[6]) {
+    const Spectral& S = *st.sp;
+    const size_t ng = S.ng, nz = S.nz, n2 = ng * ng;
+    const double *r = st.f[R], *u = st.f[U], *v = st.f[V], *w = st.f[W], *z = st.f[Z];
+    double sb = 0.0, st_ = 0.0;
+    for (size_t p = 0; p < n2; p++) sb += (1.0 + r[p]) * (std::pow(u[p], 2.0) + std::pow(v[p], 2.0));
+    double ekin = (1.0 / 2.0) * sb;
+    const size_t ot = n2 * nz;
+    for (size_t p = 0; p < n2; p++)
+        st_ += (1.0 + r[ot + p]) * (std::pow(u[ot + p], 2.0) + std::pow(v[ot + p], 2.0) + std::pow(w[ot + p], 2.0));
+    ekin += (1.0 / 2.0) * st_;
+    for (size_t iz = 1; iz < nz; iz++) {
+        double sl = 0.0;
+        const size_t o = n2 * iz;
+        for (size_t p = 0; p < n2; p++)
+            sl += (1.0 + r[o + p]) * (std::pow(u[o + p], 2.0) + std::pow(v[o + p], 2.0) + std::pow(w[o + p], 2.0));
+        ekin += sl;
+    }
+    const double gl = (2.0 * PI) / (double)ng, dz = HBAR / (double)nz, gvol = gl * gl * dz;
+    ekin *= (1.0 / 2.0) * gvol * (1.0 / HBAR);
+    double sp = 0.0;
+    for (size_t p = 0; p < n2; p++) sp += std::pow((1.0 / HBAR) * z[ot + p] - 1.0, 2.0);
+    const double epot = (1.0 / 2.0) * (gl * gl) * CSQ * sp;
+    out[0] = st.t;
+    out[1] = 0.0;
+    out[2] = ekin;
+    out[3] = ekin;
+    out[4] = epot;
+    out[5] = ekin + epot;
+}
+
+// The time loop of nhswps (src/nhswps/mod.rs:285-345) from a state whose prognostic fields qs, ds, gs
+// are set: an inversion, then while t <= tsim: at save steps (itime = round(t/dt) a multiple of
+// ngsave = round(tgsave/dt)) main_invert, psolve and the savegrid energies, ggen = false, otherwise
+// ggen = true; then advance (one monitor row).  After the loop the save test is repeated with
+// itime = floor(t/dt).  Rows beyond the given capacities are counted but not stored.
+int run(State& st, double tsim, double tgsave, int nitmax, double* mon, int maxmon, int* nmon, double* eco, int maxeco,
+        int* neco, int* iters_total) {
+    const Spectral& S = *st.sp;
+    const double dt = 1.0 / (double)S.ng;
+    double *qs = st.f[QS], *ds = st.f[DS], *gs = st.f[GS];
+    int rc, it = 0;
+    double errp = 0.0;
+    *nmon = *neco = *iters_total = 0;
+    S.main_invert(qs, ds, gs, st.f[R], st.f[U], st.f[V], st.f[ZETA]);
+    const size_t ngsave = (size_t)std::round(tgsave / dt);
+    auto save = [&]() -> int {
+        S.main_invert(qs, ds, gs, st.f[R], st.f[U], st.f[V], st.f[ZETA]);
+        if ((rc = psolve(st, nitmax, &it, &errp))) return rc;
+        *iters_total += it;
+        double e[6];
+        energy(st, e);
+        if (*neco < maxeco) std::memcpy(eco + 6 * (size_t)*neco, e, sizeof(e));
+        (*neco)++;
+        return 0;
+    };
+    while (st.t <= tsim) {
+        const size_t itime = (size_t)std::round(st.t / dt), jtime = itime / ngsave;
+        if (ngsave * jtime == itime) {
+            if ((rc = save())) return rc;
+            st.ggen = 0;
+        } else {
+            st.ggen = 1;
+        }
+        double d[5];
+        int its[3];
+        if ((rc = advance(st, nitmax, d, its))) return rc;
+        *iters_total += its[0] + its[1] + its[2];
+        if (*nmon < maxmon) std::memcpy(mon + 5 * (size_t)*nmon, d, sizeof(d));
+        (*nmon)++;
+    }
+    const size_t itime = (size_t)(st.t / dt), jtime = itime / ngsave;
+    if (ngsave * jtime == itime)
+        if ((rc = save())) return rc;
+    return 0;
+}
+
 }  // namespace orc

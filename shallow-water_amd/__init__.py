@@ -268,6 +268,27 @@ def advance(st):
     return diag, [int(i) for i in iters]
 
 
+def energy(st):
+    """The ecomp.asc row of savegrid (src/nhswps/mod.rs:365-420): [t, 0, ekin, ekin, epot, etot]."""
+    out = np.zeros(6)
+    _check(_lib.sw_energy(st._h, _d(out)))
+    return out
+
+
+def run(st, tsim, tgsave, max_rows=None):
+    """The nhswps time loop (src/nhswps/mod.rs:285-345) from the state's qs, ds, gs and time: returns
+    (monitor rows (n, 5), ecomp rows (m, 6), total psolve iterations)."""
+    dt = 1.0 / st.spectral.ng
+    if max_rows is None:
+        max_rows = int((tsim - st.t) / dt) + 4
+    mon = np.zeros((max_rows, 5))
+    eco = np.zeros((max_rows, 6))
+    nm, ne, it = C.c_int(), C.c_int(), C.c_int()
+    _check(_lib.sw_run(st._h, C.c_double(tsim), C.c_double(tgsave), _d(mon), C.c_int(max_rows), C.byref(nm), _d(eco),
+                       C.c_int(max_rows), C.byref(ne), C.byref(it)))
+    return mon[:min(nm.value, max_rows)], eco[:min(ne.value, max_rows)], it.value
+
+
 class Ensemble:
     """Member states advanced together (sw_ensemble_*): each member needs its own Spectral handle; the
     members are spread over `nstreams` HIP streams, each driven by one host thread."""

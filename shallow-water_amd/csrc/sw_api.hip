@@ -591,6 +591,37 @@ int sw_advance(sw_state* st, double diag[5], int iters[3]) {
     });
 }
 
+int sw_energy(sw_state* st, double out[6]) {
+    return guard([&] {
+        SW_CHECK(out, SW_ERR_ARG, "null output");
+        return with_state(st, [&](auto& e, auto f) {
+            e.energy(f, st->t, out);
+            return 0;
+        });
+    });
+}
+
+int sw_run(sw_state* st, double tsim, double tgsave, double* monitor, int max_monitor, int* nmonitor, double* ecomp,
+           int max_ecomp, int* necomp, int* iters_total) {
+    return guard([&] {
+        SW_CHECK(st && st->s && st->s->nz >= 3, SW_ERR_ARG, "run needs nz >= 3");
+        SW_CHECK(tgsave > 0.0 && std::round(tgsave * (double)st->s->ng) >= 1.0, SW_ERR_ARG,
+                 "save interval below one time step");
+        SW_CHECK((monitor || max_monitor == 0) && (ecomp || max_ecomp == 0), SW_ERR_ARG, "null row buffer");
+        int nm = 0, ne = 0, itot = 0;
+        int rc = with_state(st, [&](auto& e, auto f) {
+            int r = e.run(f, st->t, tsim, tgsave, monitor, max_monitor, &nm, ecomp, max_ecomp, &ne, &itot);
+            SW_HIP(hipStreamSynchronize(e.st));
+            return r;
+        });
+        st->ggen = 0;
+        if (nmonitor) *nmonitor = nm;
+        if (necomp) *necomp = ne;
+        if (iters_total) *iters_total = itot;
+        return rc;
+    });
+}
+
 int sw_ensemble_create(int nmember, sw_state* const* members, int nstreams, sw_ensemble** out) {
     return guard([&] {
         SW_CHECK(out && members && nmember > 0, SW_ERR_ARG, "bad ensemble arguments");

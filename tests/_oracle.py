@@ -24,7 +24,7 @@ def lib():
         for name in ("orc_spectral_free", "orc_spectral_get", "orc_main_invert", "orc_jacob", "orc_divs",
                      "orc_ptospc3d", "orc_spctop3d", "orc_deal3d", "orc_deal2d", "orc_spec1d",
                      "orc_vertical", "orc_coeffs", "orc_cpsource", "orc_psolve", "orc_source",
-                     "orc_diagnose", "orc_advance"):
+                     "orc_diagnose", "orc_advance", "orc_energy", "orc_run"):
             getattr(_lib, name).argtypes = None
     return _lib
 
@@ -204,3 +204,23 @@ def advance(st, nitmax=100):
                            dp(diag), iters.ctypes.data_as(C.POINTER(C.c_int)))
     st.t = t.value
     return rc, diag, [int(i) for i in iters]
+
+
+def energy(st):
+    out = np.zeros(6)
+    lib().orc_energy(st.spectral._h, st._ptrs(), C.c_double(st.t), dp(out))
+    return out
+
+
+def run(st, tsim, tgsave, nitmax=100, max_rows=None):
+    dt = 1.0 / st.spectral.ng
+    if max_rows is None:
+        max_rows = int((tsim - st.t) / dt) + 4
+    mon = np.zeros((max_rows, 5))
+    eco = np.zeros((max_rows, 6))
+    t = C.c_double(st.t)
+    nm, ne, it = C.c_int(), C.c_int(), C.c_int()
+    rc = lib().orc_run(st.spectral._h, st._ptrs(), C.byref(t), C.c_double(tsim), C.c_double(tgsave), C.c_int(nitmax),
+                       dp(mon), C.c_int(max_rows), C.byref(nm), dp(eco), C.c_int(max_rows), C.byref(ne), C.byref(it))
+    st.t = t.value
+    return rc, mon[:min(nm.value, max_rows)], eco[:min(ne.value, max_rows)], it.value
