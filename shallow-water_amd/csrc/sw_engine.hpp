@@ -596,6 +596,36 @@ struct Engine {
         SW_HIP(hipGetLastError());
     }
 
+    // A de-aliasing transform pair: the forward 2-D transform of pro (pro(s, x, y): physical value at
+    // (y, x)), each spectral coefficient multiplied by mul(s, ky, kx) (and stored at
+    // spec[s n^2 + kx + n ky] when spec is not null), then the inverse 2-D transform into epi
+    // (epi(s, x, y, v)).  On the wave path the forward pass along x, the multiply and the inverse pass
+    // along kx are one launch (fftw_fwd_mul_inv).  NW1: waves per block of the first pass (0: NWT).
+    template <int NW1 = 0, class Pro, class Mul, class Epi>
+    void fwd_mul_inv(int nslab, Pro pro, Mul mul, Epi epi, T* spec = nullptr) {
+        if (nslab <= 0) return;
+        if (wave_n()) {
+            T* t1 = tmp_for(nslab);
+            T* t2 = W("fmi", (size_t)nslab * n2);
+            wave_dispatch([&](auto nn_) {
+                constexpr int NN = decltype(nn_)::value;
+                fwd_pass1_wave<NN, NW1 ? NW1 : NWT>(nslab, pro, t1, SITE_OTHER);
+                const size_t lds = wave_lds_bytes<NN, NWT, T>();
+                LoadPlain<T> lp{t1, n2, ng};
+                hipLaunchKernelGGL((fftw_fwd_mul_inv<T, NN, NWT, LoadPlain<T>, Mul>), dim3(nslab * (NN / (2 * NWT))),
+                                   dim3(64 * NWT), lds, st, plan.tw, nslab, lp, mul, spec, t2);
+                SW_HIP(hipGetLastError());
+                inv_pass2_wave<NN>(nslab, t2, epi);
+            });
+            return;
+        }
+        T* S = spec ? spec : W("fmis", (size_t)nslab * n2);
+        const int n = ng;
+        const size_t nn = n2;
+        fwd2d<NW1>(nslab, pro, [=] __device__(int s, int ky, int kx, T v) { S[(size_t)s * nn + kx + (size_t)n * ky] = v * mul(s, ky, kx); });
+        inv2d(nslab, ld(S), epi);
+    }
+
     // plain slab accessors (level offset lev0)
     LoadPlain<T> ld(const T* a, int lev0 = 0) const { return LoadPlain<T>{a + (size_t)lev0 * n2, n2, ng}; }
     StorePlain<T> stp(T* a, int lev0 = 0) const { return StorePlain<T>{a + (size_t)lev0 * n2, n2, ng}; }
@@ -630,12 +660,9 @@ struct Engine {
 
     // deal3d / deal2d: ptospc, multiply by the 2/3-rule filter, spctop (spectral/mod.rs:524-554)
     void deal(T* fp, int nlev) {
-        T* S = W("deal", (size_t)nlev * n2);
         const T* F = filt.p;
         const int n = ng;
-        const size_t nn = n2;
-        fwd2d(nlev, ld(fp), [=] __device__(int s, int ky, int kx, T v) { S[s * nn + kx + (size_t)n * ky] = v * F[kx + (size_t)n * ky]; });
-        inv2d(nlev, ld(S), stp(fp));
+        fwd_mul_inv(nlev, ld(fp), [=] __device__(int, int ky, int kx) { return F[kx + (size_t)n * ky]; }, stp(fp));
     }
 
     void divs(const T* a, const T* b, T* cs) {  // spectral/mod.rs:461-475

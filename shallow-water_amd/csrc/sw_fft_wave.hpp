@@ -342,6 +342,73 @@ __global__ void __launch_bounds__(64 * NW) fftw_inv_comb_fwd(const C2<T>* __rest
     wave_store_trans<N, NW>(ar, ai, br, bi, reinterpret_cast<T*>(lds_raw), lane, wv, tout + (size_t)s * N * N + r0);
 }
 
+// Forward pass, spectral multiply and inverse pass along the same axis in one launch: the rows r of
+// slab s (pro(s, r, j), a real row) are transformed forward, each spectral coefficient at packed position
+// f is multiplied by mul(s, r, f) (and, when spec is not null, stored at spec[s n n + r n + f]), and the
+// rows are transformed back and stored transposed: element (r, j) at out[r + n j + n n s].  This is the
+// middle of a de-aliasing 2-D transform pair (forward pass 2, filter, inverse pass 1), with no spectral
+// intermediate in HBM.
+template <typename T, int N, int NW, class Pro, class Mul>
+__global__ void __launch_bounds__(64 * NW) fftw_fwd_mul_inv(const C2<T>* __restrict__ tw, int nslab, Pro pro, Mul mul,
+                                                           T* spec, T* tout) {
+    constexpr int E = WaveShape<N>::E, BPS = N / (2 * NW), NT = E / 2 + 1;
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    C2<T>* buf = reinterpret_cast<C2<T>*>(lds_raw) + wv * wave_region<N>();
+    const int s = blockIdx.x / BPS, r0 = (blockIdx.x - s * BPS) * 2 * NW, ra = r0 + 2 * wv;
+    if (s >= nslab) return;
+
+    C2<T> v[E];
+#pragma unroll
+    for (int r = 0; r < E; r++) {
+        const int j = lane + 64 * r;
+        v[r] = C2<T>{pro(s, ra, j), pro(s, ra + 1, j)};
+    }
+    wave_stockham<N, false>(v, buf, tw, lane);
+    T ar[NT], ai[NT], br[NT], bi[NT];
+    wave_split_pair<N>(v, buf, lane, ar, ai, br, bi);
+    HermPair<N, T> h;
+#pragma unroll
+    for (int t = 0; t < NT; t++) {
+        const int k = lane + 64 * t;
+        if (t < NT - 1 || lane == 0) {
+            const bool has_s = k > 0 && 2 * k < N;
+            h.ca[t] = ar[t] * mul(s, ra, k);
+            h.cb[t] = br[t] * mul(s, ra + 1, k);
+            h.sa[t] = has_s ? ai[t] * mul(s, ra, N - k) : T(0);
+            h.sb[t] = has_s ? bi[t] * mul(s, ra + 1, N - k) : T(0);
+            if (spec) {
+                T* o = spec + (size_t)s * N * N + (size_t)ra * N;
+                o[k] = h.ca[t];
+                o[N + k] = h.cb[t];
+                if (has_s) {
+                    o[N - k] = h.sa[t];
+                    o[2 * N - k] = h.sb[t];
+                }
+            }
+        }
+    }
+    wave_pack_herm<N>(v, buf, lane, h);
+    wave_stockham<N, true>(v, buf, tw, lane);
+    const T sc = T(1) / sqrt(T(N));
+    constexpr int RB = 2 * NW;
+    T* stg = reinterpret_cast<T*>(lds_raw);  // [x][row]
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < E; r++) {
+        const int x = lane + 64 * r;
+        stg[x * RB + 2 * wv] = v[r].x * sc;
+        stg[x * RB + 2 * wv + 1] = v[r].y * sc;
+    }
+    __syncthreads();
+    T* o = tout + (size_t)s * N * N + r0;
+#pragma unroll 4
+    for (int e = threadIdx.x; e < RB * N; e += 64 * NW) {
+        const int x = e / RB, rr = e - x * RB;
+        o[(size_t)x * N + rr] = stg[e];
+    }
+}
+
 // Forward pass along x of every level of one row, then a per-column functor across the levels.  Block
 // ky owns row ky of the LV slabs ld(l, ky, x) (l < LV); its NWV waves transform the rows as level pairs
 // (pair m on wave m mod NWV), the spectra land in an LDS column store cs[l N + kx], and each thread then
