@@ -585,12 +585,14 @@ struct Engine {
     }
     // fftw_inv_comb_fwd over nslab slab groups (transposed output); bytes = the algorithmic bytes of
     // the launch, for the profiling mode that times this site.
+    // 4-wave blocks: the kernel runs three waves per SIMD, so three blocks share a CU
+    static constexpr int NWC = 4;
     template <int NN, class Pro, class Comb>
     void inv_comb_fwd_wave(int nslab, Pro pro, Comb comb, T* out, double bytes, int site) {
-        const size_t lds_t = wave_lds_bytes<NN, NWT, T>();
+        const size_t lds_t = wave_lds_bytes<NN, NWC, T>();
         const bool tm = timed(site);
         if (tm) prof_start(bytes);
-        hipLaunchKernelGGL((fftw_inv_comb_fwd<T, NN, NWT, Pro, Comb>), dim3(nslab * (NN / (2 * NWT))), dim3(64 * NWT),
+        hipLaunchKernelGGL((fftw_inv_comb_fwd<T, NN, NWC, Pro, Comb>), dim3(nslab * (NN / (2 * NWC))), dim3(64 * NWC),
                            lds_t, st, plan.tw, nslab, pro, comb, out, gate_);
         if (tm) prof_stop();
         SW_HIP(hipGetLastError());

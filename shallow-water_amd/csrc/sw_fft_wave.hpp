@@ -315,31 +315,36 @@ __global__ void __launch_bounds__(64 * NW) fftw_inv_comb_fwd(const C2<T>* __rest
     }
     const int K = comb.count(s);
     const T sc = T(1) / sqrt(T(N));
-    // software pipeline: input k + 1 is in flight during the transform of input k (the coefficients
-    // are loaded after it: prefetching them too costs more in registers than it gains)
+    // Register budget: at most 168 VGPRs so that three waves share a SIMD (the launch uses 4-wave
+    // blocks, three per CU).  Hence no prefetch of the next input, and the lane index laundered
+    // through an empty asm in the loop and after it: otherwise the compiler hoists every lane-dependent
+    // address of the loop (loads, LDS slots, twiddles) out of it and keeps them live (~60 VGPRs).
     HermPair<N, T> h;
-    wave_fetch_herm<N>(h, lane, [&](int hh, int f) { return pro(s, 0, ra + hh, f); });
     C2<T> v[E];
 #pragma unroll 1
     for (int k = 0; k < K; k++) {
+        int ln = lane;
+        asm volatile("" : "+v"(ln));
         // an input that is the derivative along the transform axis of the previous one is packed from
-        // the previous input's registers, which are kept instead of being refilled with a fetch
-        const bool dcur = k > 0 && comb.deriv_of_prev(s, k), dnext = k + 1 < K && comb.deriv_of_prev(s, k + 1);
-        if (dcur) wave_pack_herm<N, true>(v, buf, lane, h);
-        else wave_pack_herm<N>(v, buf, lane, h);
-        if (k + 1 < K && !dnext) wave_fetch_herm<N>(h, lane, [&](int hh, int f) { return pro(s, k + 1, ra + hh, f); });
-        wave_stockham<N, true>(v, buf, tw, lane);
+        // the previous input's registers instead of being fetched
+        const bool dcur = k > 0 && comb.deriv_of_prev(s, k);
+        if (!dcur) wave_fetch_herm<N>(h, ln, [&](int hh, int f) { return pro(s, k, ra + hh, f); });
+        if (dcur) wave_pack_herm<N, true>(v, buf, ln, h);
+        else wave_pack_herm<N>(v, buf, ln, h);
+        wave_stockham<N, true>(v, buf, tw, ln);
 #pragma unroll
         for (int r = 0; r < E; r++) {
-            const int j = lane + 64 * r;
+            const int j = ln + 64 * r;
             acc[r].x = acc[r].x + comb.coef(s, k, ra, j) * (v[r].x * sc);
             acc[r].y = acc[r].y + comb.coef(s, k, ra + 1, j) * (v[r].y * sc);
         }
     }
-    wave_stockham<N, false>(acc, buf, tw, lane);
+    int lp = lane;
+    asm volatile("" : "+v"(lp));
+    wave_stockham<N, false>(acc, buf, tw, lp);
     T ar[NT], ai[NT], br[NT], bi[NT];
-    wave_split_pair<N>(acc, buf, lane, ar, ai, br, bi);
-    wave_store_trans<N, NW>(ar, ai, br, bi, reinterpret_cast<T*>(lds_raw), lane, wv, tout + (size_t)s * N * N + r0);
+    wave_split_pair<N>(acc, buf, lp, ar, ai, br, bi);
+    wave_store_trans<N, NW>(ar, ai, br, bi, reinterpret_cast<T*>(lds_raw), lp, wv, tout + (size_t)s * N * N + r0);
 }
 
 // Forward pass, spectral multiply and inverse pass along the same axis in one launch: the rows r of
