@@ -155,13 +155,44 @@ struct PsolveComb {
     }
 };
 
+// The LU factors of the pressure tridiagonal at one (kx, ky) (Spectral::new, spectral/mod.rs:214-256):
+// coefficients a0, a0b, ap, apb of rks = kx^2 + ky^2 (integer wavenumbers), then htdv(iz) = filt / (a0 +
+// ap etdv(iz-1)) and etdv(iz) = -ap htdv(iz) level by level.  Contraction is off so that the factors are
+// bitwise those of the host's IEEE arithmetic.
+struct PressureLU {
+    double a0, a0b, ap, apb, filt;
+    __device__ __forceinline__ PressureLU(double rks, double dzisq, double f) {
+#pragma clang fp contract(off)
+        a0 = -2.0 * dzisq - (5.0 / 6.0) * rks;
+        a0b = -dzisq - (1.0 / 3.0) * rks;
+        ap = dzisq - (1.0 / 12.0) * rks;
+        apb = dzisq - (1.0 / 6.0) * rks;
+        filt = f;
+    }
+    __device__ __forceinline__ double h0() const { return filt / a0b; }
+    __device__ __forceinline__ double e0(double h) const {
+#pragma clang fp contract(off)
+        return -apb * h;
+    }
+    __device__ __forceinline__ double h(double eprev) const {
+#pragma clang fp contract(off)
+        return filt / (a0 + ap * eprev);
+    }
+    __device__ __forceinline__ double e(double h) const {
+#pragma clang fp contract(off)
+        return -ap * h;
+    }
+};
+
 // psolve's column solve (psolve.rs:285-346) for one (kx, ky): the fourth-order compact tridiagonal in
-// theta with the precomputed LU (htdv, etdv, ap), in FP64, on the column c[iz n] = S(kx, ky, iz) of the
-// source spectrum; writes ps and the top level of S (used after the iteration), and returns the
-// Parseval-weighted squared change and squared old value of ps for the residual.  NZC = nz.
+// theta, in FP64, on the column c[iz n] = S(kx, ky, iz) of the source spectrum, with the LU factors
+// formed on the way down (PressureLU: no per-level factor arrays read from HBM); writes ps and the top
+// level of S (used after the iteration), and returns the Parseval-weighted squared change and squared
+// old value of ps for the residual.  NZC = nz.
 template <typename T, int NZC>
 struct PsolveCol {
-    const double *htdv, *etdv, *ap;
+    const T* filt;
+    double dzisq;
     T* ps;
     T* sptop;
     int n;
@@ -169,22 +200,27 @@ struct PsolveCol {
     __device__ __forceinline__ Acc<2> operator()(int kx, int ky, const T* c) const {
         const size_t p = kx + (size_t)n * ky;
         const double wgt = ((kx == 0 || 2 * kx == n) ? 1.0 : 2.0) * ((ky == 0 || 2 * ky == n) ? 1.0 : 2.0);
+        const double rx = 2 * kx <= n ? kx : n - kx, ry = 2 * ky <= n ? ky : n - ky;
+        const PressureLU lu(rx * rx + ry * ry, dzisq, (double)filt[p]);
         auto S = [&](int iz) { return (double)c[iz * n]; };
-        double colr[NZC];
-        double prev = ((1.0 / 3.0) * S(0) + (1.0 / 6.0) * S(1)) * htdv[p];
-        const double a = ap[p];
+        double colr[NZC], er[NZC];
+        double h = lu.h0();
+        er[0] = lu.e0(h);
+        double prev = ((1.0 / 3.0) * S(0) + (1.0 / 6.0) * S(1)) * h;
         colr[0] = prev;
 #pragma unroll
         for (int iz = 1; iz < NZC; iz++) {
+            h = lu.h(er[iz - 1]);
+            if (iz < NZC - 1) er[iz] = lu.e(h);
             const double gg = (1.0 / 12.0) * (S(iz - 1) + S(iz + 1)) + (5.0 / 6.0) * S(iz);
-            prev = (gg - a * prev) * htdv[(size_t)iz * nn + p];
+            prev = (gg - lu.ap * prev) * h;
             colr[iz] = prev;
         }
         double nxt = prev, dsum = 0.0, osum = 0.0;
 #pragma unroll
         for (int iz = NZC - 1; iz >= 0; iz--) {
             const size_t o = (size_t)iz * nn + p;
-            if (iz < NZC - 1) nxt = colr[iz] + etdv[o] * nxt;
+            if (iz < NZC - 1) nxt = colr[iz] + er[iz] * nxt;
             const T neu = T(nxt);
             const double old = (double)ps[o];
             const double d = (double)neu - old;
