@@ -39,6 +39,26 @@ struct PtrTab {
     __device__ __forceinline__ T* operator[](int i) const { return p[i]; }
 };
 
+// Slabs of fftw_deriv_lines in K groups of `per` levels: output slab s = g per + l is the derivative of
+// level l of src[g] along x (bit g of xmask set) or along y, written to level l of dst[g].
+template <typename T, int K>
+struct DerivIo {
+    PtrTab<const T, K> src;
+    PtrTab<T, K> dst;
+    int per;
+    unsigned xmask;
+    size_t nn;
+    __device__ __forceinline__ bool along_x(int s) const { return (xmask >> (s / per)) & 1u; }
+    __device__ __forceinline__ const T* in(int s) const {
+        const int g = s / per;
+        return src[g] + (size_t)(s - g * per) * nn;
+    }
+    __device__ __forceinline__ T* out(int s) const {
+        const int g = s / per;
+        return dst[g] + (size_t)(s - g * per) * nn;
+    }
+};
+
 constexpr int RED_BLOCKS = 512;
 constexpr int MAX_NZ = 256;
 
@@ -248,17 +268,22 @@ struct PsolveCtrl {
     double top2;  // squared norm of the caller's top level of pn (enters the first residual)
 };
 
-// Resets the control state; top2 = the nb block partials of its reduction summed in block order
-// (Kahan-compensated), as the host does for the other reductions.
-__global__ void k_psolve_ctrl_init(PsolveCtrl* c, const double* top2_part, int nb) {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+// One block of 256 threads: resets the control state, with top2 = the nb block partials of its
+// reduction summed as k_psolve_finalize sums the residual partials (Kahan per thread over a fixed set of
+// blocks, then a fixed shuffle tree).
+__global__ void __launch_bounds__(256) k_psolve_ctrl_init(PsolveCtrl* c, const double* top2_part, int nb) {
+    __shared__ double sh[4];
     double acc = 0.0, cmp = 0.0;
-    for (int b = 0; b < nb; b++) {
+    for (int b = threadIdx.x; b < nb; b += blockDim.x) {
         double y = top2_part[b] - cmp, t = acc + y;
         cmp = (t - acc) - y;
         acc = t;
     }
-    *c = PsolveCtrl{0, 0, 0, 0, 1.0, 1.0, acc};
+    for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    *c = PsolveCtrl{0, 0, 0, 0, 1.0, 1.0, sh[0] + sh[1] + sh[2] + sh[3]};
 }
 
 // One block of 256 threads: the residual sums from the nb block partials of the column solve (Kahan per
@@ -662,6 +687,23 @@ struct Engine {
         const size_t nn = n2;
         fwd2d<NW1>(nslab, pro, [=] __device__(int s, int ky, int kx, T v) { S[(size_t)s * nn + kx + (size_t)n * ky] = v * mul(s, ky, kx); });
         inv2d(nslab, ld(S), epi);
+    }
+
+    // Horizontal derivatives of physical slabs (DerivIo) as 1-D transform pairs along their axis
+    // (fftw_deriv_lines, one launch for all nslab outputs).  Returns false, launching nothing, when
+    // ng has no wave kernels: the caller then forms them with a 2-D transform pair.
+    template <int K>
+    bool deriv_lines(int nslab, const DerivIo<T, K>& io) {
+        if (!wave_n()) return false;
+        if (nslab <= 0) return true;
+        wave_dispatch([&](auto nn_) {
+            constexpr int NN = decltype(nn_)::value;
+            const size_t lds = deriv_lines_lds_bytes<T, NN, NWT>();
+            hipLaunchKernelGGL((fftw_deriv_lines<T, NN, NWT, DerivIo<T, K>>), dim3(nslab * (NN / (2 * NWT))),
+                               dim3(64 * NWT), lds, st, plan.tw, nslab, io);
+            SW_HIP(hipGetLastError());
+        });
+        return true;
     }
 
     // plain slab accessors (level offset lev0)

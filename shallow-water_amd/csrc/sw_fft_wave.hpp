@@ -414,6 +414,84 @@ __global__ void __launch_bounds__(64 * NW) fftw_fwd_mul_inv(const C2<T>* __restr
     }
 }
 
+// Derivative along one axis of real N x N slabs (physical layout y + N x) by a 1-D transform pair along
+// that axis alone: for output slab s the input io.in(s) is transformed forward along the axis, each
+// spectrum is replaced by its derivative (d[f] = dmul(f) x[mir(f)], wave_pack_herm<DERIV>) and
+// transformed back into io.out(s).  The transforms along the other axis of a 2-D pair cancel, so this
+// is the 2-D spectral derivative without them.  io.along_x(s) (block-uniform) picks the axis: rows x
+// (along y, contiguous) are loaded and stored straight, one pair per wave; columns y (along x) go
+// through an LDS stage [x][column] of the block's 2 NW columns, padded by two values per x so that the
+// lanes' stride-(2 NW + 2) accesses spread over the banks, and are read and written in runs of 2 NW.
+template <typename T, int N, int NW>
+constexpr size_t deriv_lines_lds_bytes() {
+    size_t ex = (size_t)NW * wave_region<N>() * sizeof(C2<T>);
+    size_t stg = (size_t)N * (2 * NW + 2) * sizeof(T);
+    return ex > stg ? ex : stg;
+}
+template <typename T, int N, int NW, class Io>
+__global__ void __launch_bounds__(64 * NW) fftw_deriv_lines(const C2<T>* __restrict__ tw, int nslab, Io io) {
+    constexpr int E = WaveShape<N>::E, BPS = N / (2 * NW), RB = 2 * NW, RS = RB + 2;
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    C2<T>* buf = reinterpret_cast<C2<T>*>(lds_raw) + wv * wave_region<N>();
+    T* stg = reinterpret_cast<T*>(lds_raw);  // column mode: [x][column], row stride RS
+    const int s = blockIdx.x / BPS, r0 = (blockIdx.x - s * BPS) * RB, ra = r0 + 2 * wv;
+    if (s >= nslab) return;
+    const bool ax = io.along_x(s);
+    const T* __restrict__ in = io.in(s);
+    T* __restrict__ out = io.out(s);
+
+    C2<T> v[E];
+    if (ax) {
+#pragma unroll 4
+        for (int e = threadIdx.x; e < RB * N; e += 64 * NW) {
+            const int x = e / RB, rr = e - x * RB;
+            stg[x * RS + rr] = in[(size_t)x * N + r0 + rr];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < E; r++) {
+            const int x = lane + 64 * r;
+            v[r] = C2<T>{stg[x * RS + 2 * wv], stg[x * RS + 2 * wv + 1]};
+        }
+        __syncthreads();  // the stage becomes the waves' exchange regions
+    } else {
+#pragma unroll
+        for (int r = 0; r < E; r++) {
+            const int j = lane + 64 * r;
+            v[r] = C2<T>{in[(size_t)ra * N + j], in[(size_t)(ra + 1) * N + j]};
+        }
+    }
+    wave_stockham<N, false>(v, buf, tw, lane);
+    HermPair<N, T> h;
+    wave_split_pair<N>(v, buf, lane, h.ca, h.sa, h.cb, h.sb);
+    wave_pack_herm<N, true>(v, buf, lane, h);
+    wave_stockham<N, true>(v, buf, tw, lane);
+    const T sc = T(1) / sqrt(T(N));
+    if (ax) {
+        __syncthreads();  // every wave is done with its exchange region
+#pragma unroll
+        for (int r = 0; r < E; r++) {
+            const int x = lane + 64 * r;
+            stg[x * RS + 2 * wv] = v[r].x * sc;
+            stg[x * RS + 2 * wv + 1] = v[r].y * sc;
+        }
+        __syncthreads();
+#pragma unroll 4
+        for (int e = threadIdx.x; e < RB * N; e += 64 * NW) {
+            const int x = e / RB, rr = e - x * RB;
+            out[(size_t)x * N + r0 + rr] = stg[x * RS + rr];
+        }
+    } else {
+#pragma unroll
+        for (int r = 0; r < E; r++) {
+            const int j = lane + 64 * r;
+            out[(size_t)ra * N + j] = v[r].x * sc;
+            out[(size_t)(ra + 1) * N + j] = v[r].y * sc;
+        }
+    }
+}
+
 // Forward pass along x of every level of one row, then a per-column functor across the levels.  Block
 // ky owns row ky of the LV slabs ld(l, ky, x) (l < LV); its NWV waves transform the rows as level pairs
 // (pair m on wave m mod NWV), the spectra land in an LDS column store cs[l N + kx], and each thread then
