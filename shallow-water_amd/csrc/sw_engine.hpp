@@ -698,7 +698,7 @@ struct Engine {
         if (nslab <= 0) return true;
         wave_dispatch([&](auto nn_) {
             constexpr int NN = decltype(nn_)::value;
-            const size_t lds = deriv_lines_lds_bytes<T, NN, NWT>();
+            const size_t lds = wave_lds_bytes<NN, NWT, T>();
             hipLaunchKernelGGL((fftw_deriv_lines<T, NN, NWT, DerivIo<T, K>>), dim3(nslab * (NN / (2 * NWT))),
                                dim3(64 * NWT), lds, st, plan.tw, nslab, io);
             SW_HIP(hipGetLastError());

@@ -47,10 +47,18 @@ __device__ __forceinline__ void wave_lds_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// Row stride of the LDS stage [f][row] through which a block of NW waves writes (or reads) 2 NW rows
+// transposed: two pad values per f, so that the lanes of a wave, which access the stage at stride
+// 2 NW + 2 values, spread over the banks instead of all hitting one.
+template <int NW>
+constexpr int stage_stride() { return 2 * NW + 2; }
+
+// Dynamic LDS of a block of NW waves: the waves' exchange regions, which a transposing block (NW > 1)
+// reuses as its stage; straight passes (NW = 1) stage nothing.
 template <int N, int NW, typename T>
 constexpr size_t wave_lds_bytes() {
     size_t ex = (size_t)NW * wave_region<N>() * sizeof(C2<T>);
-    size_t stg = (size_t)N * 2 * NW * sizeof(T);
+    size_t stg = NW > 1 ? (size_t)N * stage_stride<NW>() * sizeof(T) : 0;
     return ex > stg ? ex : stg;
 }
 
@@ -182,17 +190,18 @@ template <int N, int NW, typename T>
 __device__ __forceinline__ void wave_store_trans(const T (&ar)[WaveShape<N>::E / 2 + 1], const T (&ai)[WaveShape<N>::E / 2 + 1],
                                                  const T (&br)[WaveShape<N>::E / 2 + 1], const T (&bi)[WaveShape<N>::E / 2 + 1],
                                                  T* stg, int lane, int wv, T* o) {
-    constexpr int E = WaveShape<N>::E, NT = E / 2 + 1, RB = 2 * NW;
+    constexpr int E = WaveShape<N>::E, NT = E / 2 + 1, RB = 2 * NW, RS = stage_stride<NW>();
+    static_assert(NW > 1, "a transposed store stages 2 NW rows through the block's LDS");
     __syncthreads();
 #pragma unroll
     for (int t = 0; t < NT; t++) {
         const int k = lane + 64 * t;
         if (t < NT - 1 || lane == 0) {
-            stg[k * RB + 2 * wv] = ar[t];
-            stg[k * RB + 2 * wv + 1] = br[t];
+            stg[k * RS + 2 * wv] = ar[t];
+            stg[k * RS + 2 * wv + 1] = br[t];
             if (k > 0 && 2 * k < N) {
-                stg[(N - k) * RB + 2 * wv] = ai[t];
-                stg[(N - k) * RB + 2 * wv + 1] = bi[t];
+                stg[(N - k) * RS + 2 * wv] = ai[t];
+                stg[(N - k) * RS + 2 * wv + 1] = bi[t];
             }
         }
     }
@@ -200,7 +209,7 @@ __device__ __forceinline__ void wave_store_trans(const T (&ar)[WaveShape<N>::E /
 #pragma unroll 4
     for (int e = threadIdx.x; e < RB * N; e += 64 * NW) {
         const int f = e / RB, rr = e - f * RB;
-        o[(size_t)f * N + rr] = stg[e];
+        o[(size_t)f * N + rr] = stg[f * RS + rr];
     }
 }
 
@@ -225,7 +234,7 @@ __global__ void __launch_bounds__(64 * NW) fftw_fwd_pass(const C2<T>* __restrict
     wave_stockham<N, false>(v, buf, tw, lane);
     T ar[NT], ai[NT], br[NT], bi[NT];
     wave_split_pair<N>(v, buf, lane, ar, ai, br, bi);
-    if (!TRANS) {
+    if constexpr (!TRANS) {
 #pragma unroll
         for (int t = 0; t < NT; t++) {
             const int k = lane + 64 * t;
@@ -261,7 +270,7 @@ __global__ void __launch_bounds__(64 * NW) fftw_inv_pass(const C2<T>* __restrict
     wave_stockham<N, true>(v, buf, tw, lane);
 
     const T sc = T(1) / sqrt(T(N));
-    if (!TRANS) {
+    if constexpr (!TRANS) {
 #pragma unroll
         for (int r = 0; r < E; r++) {
             const int x = lane + 64 * r;
@@ -269,21 +278,22 @@ __global__ void __launch_bounds__(64 * NW) fftw_inv_pass(const C2<T>* __restrict
             epi(s, ra + 1, x, v[r].y * sc);
         }
     } else {
-        constexpr int RB = 2 * NW;
-        T* stg = reinterpret_cast<T*>(lds_raw);  // [x][row]
+        constexpr int RB = 2 * NW, RS = stage_stride<NW>();
+        static_assert(!TRANS || NW > 1, "a transposed store stages 2 NW rows through the block's LDS");
+        T* stg = reinterpret_cast<T*>(lds_raw);  // [x][row], row stride RS
         __syncthreads();
 #pragma unroll
         for (int r = 0; r < E; r++) {
             const int x = lane + 64 * r;
-            stg[x * RB + 2 * wv] = v[r].x * sc;
-            stg[x * RB + 2 * wv + 1] = v[r].y * sc;
+            stg[x * RS + 2 * wv] = v[r].x * sc;
+            stg[x * RS + 2 * wv + 1] = v[r].y * sc;
         }
         __syncthreads();
         T* o = tout + (size_t)s * N * N + r0;
 #pragma unroll 4
         for (int e = threadIdx.x; e < RB * N; e += 64 * NW) {
             const int x = e / RB, rr = e - x * RB;
-            o[(size_t)x * N + rr] = stg[e];
+            o[(size_t)x * N + rr] = stg[x * RS + rr];
         }
     }
 }
@@ -396,21 +406,22 @@ __global__ void __launch_bounds__(64 * NW) fftw_fwd_mul_inv(const C2<T>* __restr
     wave_pack_herm<N>(v, buf, lane, h);
     wave_stockham<N, true>(v, buf, tw, lane);
     const T sc = T(1) / sqrt(T(N));
-    constexpr int RB = 2 * NW;
-    T* stg = reinterpret_cast<T*>(lds_raw);  // [x][row]
+    constexpr int RB = 2 * NW, RS = stage_stride<NW>();
+    static_assert(NW > 1, "a transposed store stages 2 NW rows through the block's LDS");
+    T* stg = reinterpret_cast<T*>(lds_raw);  // [x][row], row stride RS
     __syncthreads();
 #pragma unroll
     for (int r = 0; r < E; r++) {
         const int x = lane + 64 * r;
-        stg[x * RB + 2 * wv] = v[r].x * sc;
-        stg[x * RB + 2 * wv + 1] = v[r].y * sc;
+        stg[x * RS + 2 * wv] = v[r].x * sc;
+        stg[x * RS + 2 * wv + 1] = v[r].y * sc;
     }
     __syncthreads();
     T* o = tout + (size_t)s * N * N + r0;
 #pragma unroll 4
     for (int e = threadIdx.x; e < RB * N; e += 64 * NW) {
         const int x = e / RB, rr = e - x * RB;
-        o[(size_t)x * N + rr] = stg[e];
+        o[(size_t)x * N + rr] = stg[x * RS + rr];
     }
 }
 
@@ -420,17 +431,12 @@ __global__ void __launch_bounds__(64 * NW) fftw_fwd_mul_inv(const C2<T>* __restr
 // transformed back into io.out(s).  The transforms along the other axis of a 2-D pair cancel, so this
 // is the 2-D spectral derivative without them.  io.along_x(s) (block-uniform) picks the axis: rows x
 // (along y, contiguous) are loaded and stored straight, one pair per wave; columns y (along x) go
-// through an LDS stage [x][column] of the block's 2 NW columns, padded by two values per x so that the
-// lanes' stride-(2 NW + 2) accesses spread over the banks, and are read and written in runs of 2 NW.
-template <typename T, int N, int NW>
-constexpr size_t deriv_lines_lds_bytes() {
-    size_t ex = (size_t)NW * wave_region<N>() * sizeof(C2<T>);
-    size_t stg = (size_t)N * (2 * NW + 2) * sizeof(T);
-    return ex > stg ? ex : stg;
-}
+// through an LDS stage [x][column] of the block's 2 NW columns (row stride stage_stride<NW>()) and are
+// read and written in runs of 2 NW.  LDS: wave_lds_bytes<N, NW, T>().
 template <typename T, int N, int NW, class Io>
 __global__ void __launch_bounds__(64 * NW) fftw_deriv_lines(const C2<T>* __restrict__ tw, int nslab, Io io) {
-    constexpr int E = WaveShape<N>::E, BPS = N / (2 * NW), RB = 2 * NW, RS = RB + 2;
+    constexpr int E = WaveShape<N>::E, BPS = N / (2 * NW), RB = 2 * NW, RS = stage_stride<NW>();
+    static_assert(NW > 1, "columns are staged through the block's LDS");
     extern __shared__ __align__(16) unsigned char lds_raw[];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     C2<T>* buf = reinterpret_cast<C2<T>*>(lds_raw) + wv * wave_region<N>();
