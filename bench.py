@@ -105,7 +105,8 @@ def measured_traffic(site, kernel, ng, nz, precision):
 
 def kernel_names(ng, nz, precision):
     t = "double" if precision == "fp64" else "float"
-    return {"psolve_deriv": f"fftw_inv_pass<{t}, {ng}, 8, true, psolve theta-derivative prologue>",
+    nwt = 4 if precision == "fp64" else 8  # waves per block of the transposing passes
+    return {"psolve_deriv": f"fftw_inv_pass<{t}, {ng}, {nwt}, true, psolve theta-derivative prologue>",
             "psolve_comb": f"fftw_inv_comb_fwd<{t}, {ng}, 4, psolve source fold>",
             "psolve_tri": f"fftw_fwd_rows_cols<{t}, {ng}, 8, {nz + 1}, psolve column solve>"}
 

@@ -590,11 +590,13 @@ struct Engine {
     }
 
     // Single passes of the wave-per-sequence kernels for n = NN.  A transposing pass groups NW waves
-    // (NWT by default) so that its store writes 2 NW consecutive values per output row (128 bytes FP64,
-    // 64 bytes FP32); a straight pass runs one wave per block.  Pass 1 of a forward transform runs along y (physical rows
-    // x) and leaves the spectrum in y transposed: out[x + n ky]; pass 2 runs along x.  Pass 1 of an
+    // (NWT by default) so that its store writes 2 NW consecutive values per output row: 4 waves in FP64
+    // (64-byte runs; with the padded stage these blocks outrun 8-wave ones, 5.9 against 5.2 TB/s for a
+    // plain 49-slab pass at n = 512), 8 in FP32 (64-byte runs again; 4-wave blocks drop to 3.5 TB/s);
+    // a straight pass runs one wave per block.  Pass 1 of a forward transform runs along y (physical
+    // rows x) and leaves the spectrum in y transposed: out[x + n ky]; pass 2 runs along x.  Pass 1 of an
     // inverse transform runs along kx (rows ky) and leaves out[ky + n x]; pass 2 runs along ky.
-    static constexpr int NWT = 8;
+    static constexpr int NWT = sizeof(T) == 8 ? 4 : 8;
     bool timed(int site) const {
         return (prof == SW_PROF_PSOLVE_DERIV && site == SITE_PSOLVE_DERIV) ||
                (prof == SW_PROF_PSOLVE_COMB && site == SITE_PSOLVE_COMB) ||
