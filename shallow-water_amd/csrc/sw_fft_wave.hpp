@@ -252,6 +252,56 @@ __global__ void __launch_bounds__(64 * NW) fftw_fwd_pass(const C2<T>* __restrict
     }
 }
 
+// Forward pass of rows that are a spectral combination of two real inputs: row r of slab s has inputs
+// pro(s, r, j) = {a_j, c_j}, transformed together as one complex sequence, split, and combined as
+// A + sig(s) D(C), D the derivative along the transform axis (d[f] = dmul(f) x[mir(f)]: cosine part
+// -k C'_k, sine part k C_k, zero at k = 0 and N/2).  A wave owns rows ra and ra + 1, one transform
+// each.  TRANS: element (r, f) goes to out[r + n f + n n s]; otherwise epi(s, r, f, value).
+template <typename T, int N, int NW, bool TRANS, class Pro, class Sig, class Epi>
+__global__ void __launch_bounds__(64 * NW) fftw_fwd_dcomb(const C2<T>* __restrict__ tw, int nslab, Pro pro, Sig sig,
+                                                         Epi epi, T* tout) {
+    constexpr int E = WaveShape<N>::E, BPS = N / (2 * NW), NT = E / 2 + 1;
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    C2<T>* buf = reinterpret_cast<C2<T>*>(lds_raw) + wv * wave_region<N>();
+    const int s = blockIdx.x / BPS, r0 = (blockIdx.x - s * BPS) * 2 * NW, ra = r0 + 2 * wv;
+    if (s >= nslab) return;
+    const T sg = sig(s);
+
+    T oc[2][NT], os[2][NT];
+#pragma unroll
+    for (int h = 0; h < 2; h++) {
+        C2<T> v[E];
+#pragma unroll
+        for (int r = 0; r < E; r++) v[r] = pro(s, ra + h, lane + 64 * r);
+        wave_stockham<N, false>(v, buf, tw, lane);
+        T ar[NT], ai[NT], cr[NT], ci[NT];
+        wave_split_pair<N>(v, buf, lane, ar, ai, cr, ci);
+#pragma unroll
+        for (int t = 0; t < NT; t++) {
+            const int k = lane + 64 * t;
+            const T m = (k == 0 || 2 * k == N) ? T(0) : sg * T(k);
+            oc[h][t] = ar[t] - m * ci[t];
+            os[h][t] = ai[t] + m * cr[t];
+        }
+    }
+    if constexpr (!TRANS) {
+#pragma unroll
+        for (int h = 0; h < 2; h++)
+#pragma unroll
+            for (int t = 0; t < NT; t++) {
+                const int k = lane + 64 * t;
+                if (t < NT - 1 || lane == 0) {
+                    epi(s, ra + h, k, oc[h][t]);
+                    if (k > 0 && 2 * k < N) epi(s, ra + h, N - k, os[h][t]);
+                }
+            }
+    } else {
+        wave_store_trans<N, NW>(oc[0], os[0], oc[1], os[1], reinterpret_cast<T*>(lds_raw), lane, wv,
+                                tout + (size_t)s * N * N + r0);
+    }
+}
+
 // Inverse (Hermitian -> real) pass; pro(s, r, f) gives the Hermitian row r at packed position f.
 // TRANS: element (r, x) goes to out[r + n x + n n s]; otherwise epi(s, r, x, value).
 template <typename T, int N, int NW, bool TRANS, class Pro, class Epi>
