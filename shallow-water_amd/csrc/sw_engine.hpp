@@ -528,6 +528,22 @@ struct Engine {
         }
     }
 
+    // f(integral_constant<int, nz>) when nz has a compile-time specialisation; false (f not called)
+    // otherwise.  The per-point level loops use it to hold a column in registers and to issue every
+    // level's loads before the first store (the compiler may not move loads past stores to arrays that
+    // might alias, which otherwise costs one memory latency per level).
+    template <class F>
+    bool with_nz(F f) {
+        bool done = false;
+        nz_dispatch([&](auto nzc) {
+            if constexpr (decltype(nzc)::value > 0) {
+                f(nzc);
+                done = true;
+            }
+        });
+        return done;
+    }
+
     // Forward 2-D transform of nslab slabs: pro(s, x, y) gives the physical value at (y, x);
     // epi(s, ky, kx, v) receives the spectral coefficient at (kx, ky).
     // NW: waves per block of the transposing pass (0: NWT).
