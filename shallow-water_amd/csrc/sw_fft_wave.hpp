@@ -48,10 +48,11 @@ __device__ __forceinline__ void wave_lds_sync() {
 }
 
 // Row stride of the LDS stage [f][row] through which a block of NW waves writes (or reads) 2 NW rows
-// transposed: two pad values per f, so that the lanes of a wave, which access the stage at stride
-// 2 NW + 2 values, spread over the banks instead of all hitting one.
+// transposed: one pad value per f, so that the lanes of a wave, which access the stage at an odd
+// stride of 2 NW + 1 values, spread over the banks instead of all hitting one, and the stage fits in
+// the block's exchange regions (2 NW + 1 <= 18 N / 8 values per wave).
 template <int NW>
-constexpr int stage_stride() { return 2 * NW + 2; }
+constexpr int stage_stride() { return 2 * NW + 1; }
 
 // Dynamic LDS of a block of NW waves: the waves' exchange regions, which a transposing block (NW > 1)
 // reuses as its stage; straight passes (NW = 1) stage nothing.
