@@ -275,8 +275,9 @@ def main():
               f"+ write {args.copy_calibration} MiB each)", file=sys.stderr, flush=True)
         return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py measures one GPU only: this build has no multi-GPU path (members or levels would "
-              "shard across GPUs as described in DESIGN.md); run with --gpus 1", file=sys.stderr)
+        print("bench.py measures one GPU only: the library has no multi-GPU path (ensemble members, or the "
+              "levels of one member with an all-to-all per psolve iteration, would shard across GPUs); "
+              "run with --gpus 1", file=sys.stderr)
         sys.exit(2)
     sw = load_sw()
     ng, nz = args.ng, args.nz
