@@ -346,6 +346,11 @@ struct Engine {
     size_t lds_bytes = 0;
     DevBuf<C2<T>> tw;
     DevBuf<T> hlap, glap, rlap, c2g2, simp, rope, fope, filt, diss, rdis;
+    // Rows of the de-aliasing multipliers (filt, glap) are zero at every packed ky whose wavenumber
+    // min(ky, ng - ky) exceeds band_ (the 2/3 rule's circle, |k| > ng/3, contains those whole rows):
+    // the de-aliasing transform pairs neither store nor read them.  ng / 2 (no such rows) unless the
+    // operators are seen to vanish there.
+    int band_ = 0;
     DevBuf<double> ap, htdv, etdv;  // FP64 always: the pressure tridiagonal
     double weight[MAX_NZ + 1], theta[MAX_NZ + 1], htd1[MAX_NZ], etd1[MAX_NZ];
     DevBuf<double> vc;  // device copy: weight[0..L), theta[L..2L), htd1[2L..2L+nz), etd1[2L+nz..2L+2nz)
@@ -435,6 +440,18 @@ struct Engine {
         up(hlap, host.hlap); up(glap, host.glap); up(rlap, host.rlap); up(c2g2, host.c2g2);
         up(simp, host.simp); up(rope, host.rope); up(fope, host.fope); up(filt, host.filt);
         up(diss, host.diss); up(rdis, host.rdis);
+        {
+            const int kc = ng / 3;
+            bool zero = true;
+            for (int ky = 0; ky < ng && zero; ky++)
+                if (std::min(ky, ng - ky) > kc)
+                    for (int kx = 0; kx < ng; kx++)
+                        if (host.filt[(size_t)kx + (size_t)ng * ky] != 0.0 || host.glap[(size_t)kx + (size_t)ng * ky] != 0.0) {
+                            zero = false;
+                            break;
+                        }
+            band_ = zero ? kc : ng / 2;
+        }
         upd(ap, host.ap); upd(htdv, host.htdv); upd(etdv, host.etdv);
         for (int i = 0; i <= nz; i++) weight[i] = host.weight[i], theta[i] = host.theta[i];
         for (int i = 0; i < nz; i++) htd1[i] = host.htd1[i], etd1[i] = host.etd1[i];
@@ -618,13 +635,14 @@ struct Engine {
                (prof == SW_PROF_PSOLVE_COMB && site == SITE_PSOLVE_COMB) ||
                (prof == SW_PROF_PSOLVE_TRI && site == SITE_PSOLVE_TRI);
     }
+    // kcut (< 0: NN / 2, every row): output rows whose wavenumber exceeds it are not stored
     template <int NN, int NW, class Pro>
-    void fwd_pass1_wave(int nslab, Pro pro, T* out, int site) {
+    void fwd_pass1_wave(int nslab, Pro pro, T* out, int site, int kcut = -1) {
         const size_t lds_t = wave_lds_bytes<NN, NW, T>();
         const bool tm = timed(site);
         if (tm) prof_start(2.0 * sizeof(T) * (double)n2 * nslab);
         hipLaunchKernelGGL((fftw_fwd_pass<T, NN, NW, true, Pro, NoEpi>), dim3(nslab * (NN / (2 * NW))), dim3(64 * NW),
-                           lds_t, st, plan.tw, nslab, pro, NoEpi{}, out);
+                           lds_t, st, plan.tw, nslab, pro, NoEpi{}, out, kcut < 0 ? NN / 2 : kcut);
         if (tm) prof_stop();
         SW_HIP(hipGetLastError());
     }
@@ -635,7 +653,7 @@ struct Engine {
         if (tm) prof_start(2.0 * sizeof(T) * (double)n2 * nslab);
         LoadPlain<T> lp{in, n2, ng};
         hipLaunchKernelGGL((fftw_fwd_pass<T, NN, 1, false, LoadPlain<T>, Epi>), dim3(nslab * (NN / 2)), dim3(64),
-                           lds_s, st, plan.tw, nslab, lp, epi, (T*)nullptr);
+                           lds_s, st, plan.tw, nslab, lp, epi, (T*)nullptr, NN / 2);
         if (tm) prof_stop();
         SW_HIP(hipGetLastError());
     }
@@ -647,18 +665,20 @@ struct Engine {
         const bool tm = timed(site);
         if (tm) prof_start(bytes > 0.0 ? bytes : 2.0 * sizeof(T) * (double)n2 * nslab);
         hipLaunchKernelGGL((fftw_inv_pass<T, NN, NW, true, Pro, NoEpi>), dim3(nslab * (NN / (2 * NW))), dim3(64 * NW),
-                           lds_t, st, plan.tw, nslab, pro, NoEpi{}, out, gate_);
+                           lds_t, st, plan.tw, nslab, pro, NoEpi{}, out, gate_, NN / 2);
         if (tm) prof_stop();
         SW_HIP(hipGetLastError());
     }
+    // kcut (< 0: NN / 2, every position): input positions whose wavenumber exceeds it are taken as
+    // zero and not read
     template <int NN, class Epi>
-    void inv_pass2_wave(int nslab, const T* in, Epi epi) {
+    void inv_pass2_wave(int nslab, const T* in, Epi epi, int kcut = -1) {
         const size_t lds_s = wave_lds_bytes<NN, 1, T>();
         const bool tm = prof == SW_PROF_STRAIGHT;
         if (tm) prof_start(2.0 * sizeof(T) * (double)n2 * nslab);
         LoadPlain<T> lp{in, n2, ng};
         hipLaunchKernelGGL((fftw_inv_pass<T, NN, 1, false, LoadPlain<T>, Epi>), dim3(nslab * (NN / 2)), dim3(64),
-                           lds_s, st, plan.tw, nslab, lp, epi, (T*)nullptr, gate_);
+                           lds_s, st, plan.tw, nslab, lp, epi, (T*)nullptr, gate_, kcut < 0 ? NN / 2 : kcut);
         if (tm) prof_stop();
         SW_HIP(hipGetLastError());
     }
@@ -682,21 +702,24 @@ struct Engine {
     // spec[s n^2 + kx + n ky] when spec is not null), then the inverse 2-D transform into epi
     // (epi(s, x, y, v)).  On the wave path the forward pass along x, the multiply and the inverse pass
     // along kx are one launch (fftw_fwd_mul_inv).  NW1: waves per block of the first pass (0: NWT).
+    // kcut: mul is zero on every row ky whose wavenumber exceeds it (band_ for the filter and glap;
+    // < 0: no such rows); those rows are neither stored nor read between the three launches.
     template <int NW1 = 0, class Pro, class Mul, class Epi>
-    void fwd_mul_inv(int nslab, Pro pro, Mul mul, Epi epi, T* spec = nullptr) {
+    void fwd_mul_inv(int nslab, Pro pro, Mul mul, Epi epi, T* spec = nullptr, int kcut = -1) {
         if (nslab <= 0) return;
         if (wave_n()) {
             T* t1 = tmp_for(nslab);
             T* t2 = W("fmi", (size_t)nslab * n2);
             wave_dispatch([&](auto nn_) {
                 constexpr int NN = decltype(nn_)::value;
-                fwd_pass1_wave<NN, NW1 ? NW1 : NWT>(nslab, pro, t1, SITE_OTHER);
+                const int kc = kcut < 0 ? NN / 2 : kcut;
+                fwd_pass1_wave<NN, NW1 ? NW1 : NWT>(nslab, pro, t1, SITE_OTHER, kc);
                 const size_t lds = wave_lds_bytes<NN, NWT, T>();
                 LoadPlain<T> lp{t1, n2, ng};
                 hipLaunchKernelGGL((fftw_fwd_mul_inv<T, NN, NWT, LoadPlain<T>, Mul>), dim3(nslab * (NN / (2 * NWT))),
-                                   dim3(64 * NWT), lds, st, plan.tw, nslab, lp, mul, spec, t2);
+                                   dim3(64 * NWT), lds, st, plan.tw, nslab, lp, mul, spec, t2, kc);
                 SW_HIP(hipGetLastError());
-                inv_pass2_wave<NN>(nslab, t2, epi);
+                inv_pass2_wave<NN>(nslab, t2, epi, kc);
             });
             return;
         }
@@ -760,7 +783,7 @@ struct Engine {
     void deal(T* fp, int nlev) {
         const T* F = filt.p;
         const int n = ng;
-        fwd_mul_inv(nlev, ld(fp), [=] __device__(int, int ky, int kx) { return F[kx + (size_t)n * ky]; }, stp(fp));
+        fwd_mul_inv(nlev, ld(fp), [=] __device__(int, int ky, int kx) { return F[kx + (size_t)n * ky]; }, stp(fp), nullptr, band_);
     }
 
     void divs(const T* a, const T* b, T* cs) {  // spectral/mod.rs:461-475
