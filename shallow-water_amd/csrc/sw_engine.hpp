@@ -635,14 +635,14 @@ struct Engine {
                (prof == SW_PROF_PSOLVE_COMB && site == SITE_PSOLVE_COMB) ||
                (prof == SW_PROF_PSOLVE_TRI && site == SITE_PSOLVE_TRI);
     }
-    // kcut (< 0: NN / 2, every row): output rows whose wavenumber exceeds it are not stored
-    template <int NN, int NW, class Pro>
-    void fwd_pass1_wave(int nslab, Pro pro, T* out, int site, int kcut = -1) {
+    // CUT: output rows whose wavenumber exceeds kcut are not stored
+    template <int NN, int NW, bool CUT = false, class Pro>
+    void fwd_pass1_wave(int nslab, Pro pro, T* out, int site, int kcut = NN / 2) {
         const size_t lds_t = wave_lds_bytes<NN, NW, T>();
         const bool tm = timed(site);
         if (tm) prof_start(2.0 * sizeof(T) * (double)n2 * nslab);
-        hipLaunchKernelGGL((fftw_fwd_pass<T, NN, NW, true, Pro, NoEpi>), dim3(nslab * (NN / (2 * NW))), dim3(64 * NW),
-                           lds_t, st, plan.tw, nslab, pro, NoEpi{}, out, kcut < 0 ? NN / 2 : kcut);
+        hipLaunchKernelGGL((fftw_fwd_pass<T, NN, NW, true, Pro, NoEpi, CUT>), dim3(nslab * (NN / (2 * NW))), dim3(64 * NW),
+                           lds_t, st, plan.tw, nslab, pro, NoEpi{}, out, kcut);
         if (tm) prof_stop();
         SW_HIP(hipGetLastError());
     }
@@ -669,16 +669,15 @@ struct Engine {
         if (tm) prof_stop();
         SW_HIP(hipGetLastError());
     }
-    // kcut (< 0: NN / 2, every position): input positions whose wavenumber exceeds it are taken as
-    // zero and not read
-    template <int NN, class Epi>
-    void inv_pass2_wave(int nslab, const T* in, Epi epi, int kcut = -1) {
+    // CUT: input positions whose wavenumber exceeds kcut are taken as zero and not read
+    template <int NN, bool CUT = false, class Epi>
+    void inv_pass2_wave(int nslab, const T* in, Epi epi, int kcut = NN / 2) {
         const size_t lds_s = wave_lds_bytes<NN, 1, T>();
         const bool tm = prof == SW_PROF_STRAIGHT;
         if (tm) prof_start(2.0 * sizeof(T) * (double)n2 * nslab);
         LoadPlain<T> lp{in, n2, ng};
-        hipLaunchKernelGGL((fftw_inv_pass<T, NN, 1, false, LoadPlain<T>, Epi>), dim3(nslab * (NN / 2)), dim3(64),
-                           lds_s, st, plan.tw, nslab, lp, epi, (T*)nullptr, gate_, kcut < 0 ? NN / 2 : kcut);
+        hipLaunchKernelGGL((fftw_inv_pass<T, NN, 1, false, LoadPlain<T>, Epi, CUT>), dim3(nslab * (NN / 2)), dim3(64),
+                           lds_s, st, plan.tw, nslab, lp, epi, (T*)nullptr, gate_, kcut);
         if (tm) prof_stop();
         SW_HIP(hipGetLastError());
     }
@@ -713,13 +712,13 @@ struct Engine {
             wave_dispatch([&](auto nn_) {
                 constexpr int NN = decltype(nn_)::value;
                 const int kc = kcut < 0 ? NN / 2 : kcut;
-                fwd_pass1_wave<NN, NW1 ? NW1 : NWT>(nslab, pro, t1, SITE_OTHER, kc);
+                fwd_pass1_wave<NN, NW1 ? NW1 : NWT, true>(nslab, pro, t1, SITE_OTHER, kc);
                 const size_t lds = wave_lds_bytes<NN, NWT, T>();
                 LoadPlain<T> lp{t1, n2, ng};
                 hipLaunchKernelGGL((fftw_fwd_mul_inv<T, NN, NWT, LoadPlain<T>, Mul>), dim3(nslab * (NN / (2 * NWT))),
                                    dim3(64 * NWT), lds, st, plan.tw, nslab, lp, mul, spec, t2, kc);
                 SW_HIP(hipGetLastError());
-                inv_pass2_wave<NN>(nslab, t2, epi, kc);
+                inv_pass2_wave<NN, true>(nslab, t2, epi, kc);
             });
             return;
         }

@@ -107,15 +107,15 @@ struct HermPair {
     static constexpr int NT = WaveShape<N>::E / 2 + 1;
     T ca[NT], cb[NT], sa[NT], sb[NT];
 };
-// Wavenumbers k > kcut are taken as zero and not loaded (kcut = N / 2 loads every one).
-template <int N, typename T, class Ld>
+// CUT: wavenumbers k > kcut are taken as zero and not loaded.
+template <int N, bool CUT = false, typename T, class Ld>
 __device__ __forceinline__ void wave_fetch_herm(HermPair<N, T>& h, int lane, Ld ld, int kcut = N / 2) {
     constexpr int NT = HermPair<N, T>::NT;
 #pragma unroll
     for (int t = 0; t < NT; t++) {
         const int k = lane + 64 * t;
         if (t < NT - 1 || lane == 0) {
-            const bool in = k <= kcut, has_s = in && k > 0 && 2 * k < N;
+            const bool in = !CUT || k <= kcut, has_s = in && k > 0 && 2 * k < N;
             h.ca[t] = in ? ld(0, k) : T(0);
             h.cb[t] = in ? ld(1, k) : T(0);
             h.sa[t] = has_s ? ld(0, N - k) : T(0);
@@ -151,11 +151,11 @@ __device__ __forceinline__ void wave_pack_herm(C2<T> (&v)[WaveShape<N>::E], C2<T
 #pragma unroll
     for (int r = 0; r < E; r++) v[r] = buf[wpad(lane + 64 * r)];
 }
-template <int N, typename T, class Ld>
+template <int N, bool CUT = false, typename T, class Ld>
 __device__ __forceinline__ void wave_load_herm_pair(C2<T> (&v)[WaveShape<N>::E], C2<T>* buf, int lane, Ld ld,
                                                     int kcut = N / 2) {
     HermPair<N, T> h;
-    wave_fetch_herm<N>(h, lane, ld, kcut);
+    wave_fetch_herm<N, CUT>(h, lane, ld, kcut);
     wave_pack_herm<N>(v, buf, lane, h);
 }
 
@@ -189,9 +189,9 @@ __device__ __forceinline__ void wave_split_pair(const C2<T> (&v)[WaveShape<N>::E
 // Transposed store of the 2 NW rows r0 .. r0 + 2 NW - 1 of a block (wave wv holds rows r0 + 2 wv and
 // r0 + 2 wv + 1 as split by wave_split_pair): element (r, f) goes to o[r + N f], written through an LDS
 // stage [f][row] so that each output row gets a run of 2 NW consecutive values.  o = slab base + r0.
-// Output rows f whose wavenumber min(f, N - f) exceeds kcut are not stored (a band that the next pass
-// treats as zero; kcut = N / 2 stores every row).
-template <int N, int NW, typename T>
+// CUT: output rows f whose wavenumber min(f, N - f) exceeds kcut are not stored (a band that the next
+// pass treats as zero).
+template <int N, int NW, bool CUT = false, typename T>
 __device__ __forceinline__ void wave_store_trans(const T (&ar)[WaveShape<N>::E / 2 + 1], const T (&ai)[WaveShape<N>::E / 2 + 1],
                                                  const T (&br)[WaveShape<N>::E / 2 + 1], const T (&bi)[WaveShape<N>::E / 2 + 1],
                                                  T* stg, int lane, int wv, T* o, int kcut = N / 2) {
@@ -211,23 +211,31 @@ __device__ __forceinline__ void wave_store_trans(const T (&ar)[WaveShape<N>::E /
         }
     }
     __syncthreads();
-    const int f1 = kcut + 1, f2 = N - kcut > f1 ? N - kcut : f1;  // rows [0, f1) and [f2, N)
+    if constexpr (CUT) {
+        const int f1 = kcut + 1, f2 = N - kcut > f1 ? N - kcut : f1;  // rows [0, f1) and [f2, N)
 #pragma unroll 4
-    for (int e = threadIdx.x; e < f1 * RB; e += 64 * NW) {
-        const int f = e / RB, rr = e - f * RB;
-        o[(size_t)f * N + rr] = stg[f * RS + rr];
-    }
+        for (int e = threadIdx.x; e < f1 * RB; e += 64 * NW) {
+            const int f = e / RB, rr = e - f * RB;
+            o[(size_t)f * N + rr] = stg[f * RS + rr];
+        }
 #pragma unroll 4
-    for (int e = f2 * RB + threadIdx.x; e < N * RB; e += 64 * NW) {
-        const int f = e / RB, rr = e - f * RB;
-        o[(size_t)f * N + rr] = stg[f * RS + rr];
+        for (int e = f2 * RB + threadIdx.x; e < N * RB; e += 64 * NW) {
+            const int f = e / RB, rr = e - f * RB;
+            o[(size_t)f * N + rr] = stg[f * RS + rr];
+        }
+    } else {
+#pragma unroll 4
+        for (int e = threadIdx.x; e < RB * N; e += 64 * NW) {
+            const int f = e / RB, rr = e - f * RB;
+            o[(size_t)f * N + rr] = stg[f * RS + rr];
+        }
     }
 }
 
 // Forward (real -> Hermitian) pass; pro(s, r, j) gives row r, element j.  TRANS: element (r, f) goes
-// to out[r + n f + n n s], rows f with min(f, N - f) > kcut left unwritten (see wave_store_trans);
-// otherwise epi(s, r, f, value).
-template <typename T, int N, int NW, bool TRANS, class Pro, class Epi>
+// to out[r + n f + n n s] (with CUT, rows f with min(f, N - f) > kcut left unwritten: see
+// wave_store_trans); otherwise epi(s, r, f, value).
+template <typename T, int N, int NW, bool TRANS, class Pro, class Epi, bool CUT = false>
 __global__ void __launch_bounds__(64 * NW) fftw_fwd_pass(const C2<T>* __restrict__ tw, int nslab, Pro pro, Epi epi,
                                                         T* tout, int kcut) {
     constexpr int E = WaveShape<N>::E, BPS = N / (2 * NW), NT = E / 2 + 1;
@@ -260,8 +268,8 @@ __global__ void __launch_bounds__(64 * NW) fftw_fwd_pass(const C2<T>* __restrict
             }
         }
     } else {
-        wave_store_trans<N, NW>(ar, ai, br, bi, reinterpret_cast<T*>(lds_raw), lane, wv, tout + (size_t)s * N * N + r0,
-                                kcut);
+        wave_store_trans<N, NW, CUT>(ar, ai, br, bi, reinterpret_cast<T*>(lds_raw), lane, wv,
+                                     tout + (size_t)s * N * N + r0, kcut);
     }
 }
 
@@ -315,10 +323,10 @@ __global__ void __launch_bounds__(64 * NW) fftw_fwd_dcomb(const C2<T>* __restric
     }
 }
 
-// Inverse (Hermitian -> real) pass; pro(s, r, f) gives the Hermitian row r at packed position f, the
-// positions of wavenumbers above kcut taken as zero and not read (kcut = N / 2 reads every one).
-// TRANS: element (r, x) goes to out[r + n x + n n s]; otherwise epi(s, r, x, value).
-template <typename T, int N, int NW, bool TRANS, class Pro, class Epi>
+// Inverse (Hermitian -> real) pass; pro(s, r, f) gives the Hermitian row r at packed position f (with
+// CUT, the positions of wavenumbers above kcut taken as zero and not read).  TRANS: element (r, x) goes
+// to out[r + n x + n n s]; otherwise epi(s, r, x, value).
+template <typename T, int N, int NW, bool TRANS, class Pro, class Epi, bool CUT = false>
 __global__ void __launch_bounds__(64 * NW) fftw_inv_pass(const C2<T>* __restrict__ tw, int nslab, Pro pro, Epi epi,
                                                         T* tout, const int* gate, int kcut) {
     if (gate && *gate) return;  // a finished device-controlled iteration (see k_psolve_finalize)
@@ -330,7 +338,7 @@ __global__ void __launch_bounds__(64 * NW) fftw_inv_pass(const C2<T>* __restrict
     if (s >= nslab) return;
 
     C2<T> v[E];
-    wave_load_herm_pair<N>(v, buf, lane, [&](int h, int f) { return pro(s, ra + h, f); }, kcut);
+    wave_load_herm_pair<N, CUT>(v, buf, lane, [&](int h, int f) { return pro(s, ra + h, f); }, kcut);
     wave_stockham<N, true>(v, buf, tw, lane);
 
     const T sc = T(1) / sqrt(T(N));
