@@ -137,7 +137,7 @@ __global__ void __launch_bounds__(256) k_level_sum2(size_t n2, F f, double* part
 // Mean flow of main_invert (spectral/mod.rs:409-416): out[c] = -sum_iz w_iz * S_iz,c / ng^2, with
 // the level sums S taken in a fixed order from the partials (deterministic): one thread per (level,
 // component) sums the nb block partials in block order, then the levels are combined in order.
-__global__ void __launch_bounds__(256) k_meanflow_finalize(const double* part, int nb, int nlev, const double* w,
+static __global__ void __launch_bounds__(256) k_meanflow_finalize(const double* part, int nb, int nlev, const double* w,
                                                            double dsumi, double* out) {
     __shared__ double ls[2 * (MAX_NZ + 1)];
     for (int t = threadIdx.x; t < 2 * nlev; t += blockDim.x) {
@@ -271,7 +271,7 @@ struct PsolveCtrl {
 // One block of 256 threads: resets the control state, with top2 = the nb block partials of its
 // reduction summed as k_psolve_finalize sums the residual partials (Kahan per thread over a fixed set of
 // blocks, then a fixed shuffle tree).
-__global__ void __launch_bounds__(256) k_psolve_ctrl_init(PsolveCtrl* c, const double* top2_part, int nb) {
+static __global__ void __launch_bounds__(256) k_psolve_ctrl_init(PsolveCtrl* c, const double* top2_part, int nb) {
     __shared__ double sh[4];
     double acc = 0.0, cmp = 0.0;
     for (int b = threadIdx.x; b < nb; b += blockDim.x) {
@@ -292,7 +292,7 @@ __global__ void __launch_bounds__(256) k_psolve_ctrl_init(PsolveCtrl* c, const d
 // (c->top2); non-finite or (after the first iteration) errp > 1 stop with an error; errp <= toler or
 // nitmax iterations stop; in the mixed mode also two consecutive iterations from the third on whose
 // residual failed to halve while below 1e-4.
-__global__ void __launch_bounds__(256) k_psolve_finalize(const double* part, int nb, double toler, int nitmax,
+static __global__ void __launch_bounds__(256) k_psolve_finalize(const double* part, int nb, double toler, int nitmax,
                                                          int mixed, PsolveCtrl* c) {
     __shared__ double sh[4][2];
     if (c->done) return;
@@ -816,7 +816,31 @@ struct Engine {
         }, stp(cs));
     }
 
-#include "sw_engine_ops.inc"
+    // ---------------------------------------------------------------- nhswps and the time loop
+    // Defined in sw_engine_ops.inc and instantiated once per precision (sw_engine_f64.hip,
+    // sw_engine_f32.hip).
+    struct Fields {
+        T* a[SW_NFIELDS];
+    };
+    void main_invert(const T* qs, const T* ds, const T* gs, T* r, T* u, T* v, T* zeta);
+    void vertical(Fields f);
+    void coeffs(Fields f, T* sigx, T* sigy, T* cpt1, T* cpt2);
+    void cpsource(Fields f, T* sp0);
+    T* psolve_source_wave(const T* ps, const T* sp0, const T* sigx, const T* sigy, const T* cpt1, const T* cpt2);
+    bool psolve_solve_wave(const T* SPt, T* SP, T* ps, double* sums);
+    void psolve_iterate_device(const T* sp0, const T* sigx, const T* sigy, const T* cpt1, const T* cpt2, T* SP, T* ps,
+                               const T* pn, int nitmax, int* iters, double* errp, int* status);
+    int psolve(Fields f, int nitmax, int* iters_out, double* errp_out);
+    void source(Fields f, T* sqs, T* sds, T* sgs, bool reuse_derivs = false);
+    void diagnose(Fields f, double t, double out[5]);
+    void energy(Fields f, double t, double out[6]);
+    int run(Fields f, double& t, double tsim, double tgsave, double* mon, int maxmon, int* nmon, double* eco, int maxeco,
+            int* neco, int* iters_total);
+    int advance(Fields f, double& t, bool ggen, double diag[5], int iters[3]);
 };
+
+// The two precisions are instantiated in their own translation units.
+extern template struct Engine<double>;
+extern template struct Engine<float>;
 
 }  // namespace sw

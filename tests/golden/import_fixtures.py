@@ -1,7 +1,10 @@
 """Copy the reference crate's golden vectors for the nhswps hot path into tests/golden/ref.
 
 The fixtures are data files held by the reference's own tests (big-endian f64, Fortran order,
-no header; shapes encoded in the file name).  This script copies them verbatim, converts the
+no header; shapes encoded in the file name), plus the end-to-end case of src/test.rs:27-36 for
+ng = 32, nz = 4: its balanced initial state (testdata/balinit/32_4_sw_init.r8, three little-endian
+f64 records of an 8-byte header and ng^2 values: q, delta, gamma) and the run's outputs
+(testdata/complete/32_4: monitor.asc, ecomp.asc and the .r4 field dumps).  This script copies them verbatim, converts the
 insta snapshot values (factor lists and wavenumber arrays) into snapshots.json, and writes
 MANIFEST.json with the SHA-256 and source path of every file.  It is the only script that reads
 the reference tree; nothing at test time does.
@@ -25,6 +28,12 @@ DIRS = [
     "nhswps/testdata/psolve", "nhswps/testdata/advance/18_6", "nhswps/testdata/advance/24_4",
 ]
 
+# single files of the end-to-end case (src/test.rs:27-36)
+EXTRA = ["testdata/balinit/32_4_sw_init.r8", "testdata/complete/32_4/monitor.asc",
+         "testdata/complete/32_4/ecomp.asc"] + \
+        [f"testdata/complete/32_4/2d/{f}.r4" for f in ("d", "g", "h", "q", "zeta")] + \
+        [f"testdata/complete/32_4/3d/{f}.r4" for f in ("d", "g", "pn", "ql", "r", "w")]
+
 
 def snap_values(path):
     txt = open(path).read().split("---", 2)[2]
@@ -46,6 +55,13 @@ def main(ref):
             data = open(os.path.join(dst, f), "rb").read()
             manifest[f"{d}/{f}"] = {"sha256": hashlib.sha256(data).hexdigest(),
                                     "source": f"src/{d}/{f}", "bytes": len(data)}
+    for rel in EXTRA:
+        src = os.path.join(ref, "src", rel)
+        dst = os.path.join(out, rel)
+        os.makedirs(os.path.dirname(dst), exist_ok=True)
+        shutil.copyfile(src, dst)
+        data = open(dst, "rb").read()
+        manifest[rel] = {"sha256": hashlib.sha256(data).hexdigest(), "source": f"src/{rel}", "bytes": len(data)}
     snaps = {}
     for sd in ("src/stafft/snapshots", "src/snapshots"):
         for f in sorted(os.listdir(os.path.join(ref, sd))):
