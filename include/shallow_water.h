@@ -62,6 +62,13 @@ int sw_d2fft_create(size_t nx, size_t ny, double lx, double ly, double* kx, doub
 int sw_d2fft_destroy(sw_d2fft* d);
 int sw_ptospc(const sw_d2fft* d, double* rvar, double* svar);         /* ptospc  :91  */
 int sw_spctop(const sw_d2fft* d, double* svar, double* rvar);         /* spctop  :113 */
+/* ptospc without any allocation: no counterpart in the reference.  For callers that cannot allocate
+ * device or host memory during the call (code between graph captures, tight serving loops, harnesses
+ * that forbid hipMalloc): rvar and svar must be device arrays; the buffer between the two passes is a
+ * fixed scratch array that the library holds on each device (one slab of 512 x 512 doubles, 2 MiB), so
+ * nx <= 512 (SW_ERR_ARG otherwise).  Every call on a device shares that scratch: calls that overlap on
+ * one device race.  The result equals sw_ptospc's bit for bit. */
+int sw_ptospc_noalloc(const sw_d2fft* d, const double* rvar, double* svar);
 int sw_xderiv(const sw_d2fft* d, const double* rkx, const double* var, double* der); /* :128 */
 int sw_yderiv(const sw_d2fft* d, const double* rky, const double* var, double* der); /* :157 */
 

@@ -95,6 +95,32 @@ struct Arg {
     }
 };
 
+// Scratch of sw_ptospc_noalloc: the intermediate slab between the two passes of its 2-D transform,
+// for grids up to NOALLOC_MAX_NG x NOALLOC_MAX_NG (2 MiB, held by every device the library runs on
+// until the process ends).  Each device has its own copy at its own address: host code never uses
+// the array's name as an address, only the pointer hipGetSymbolAddress gives for the current device
+// (scratch_on_device()).  All calls on one device share it, so such calls must not overlap.
+constexpr size_t NOALLOC_MAX_NG = 512;
+static __device__ __align__(256) double scratch[NOALLOC_MAX_NG * NOALLOC_MAX_NG];
+constexpr int MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[MAX_DEVICES] = {};
+
+// The address of this device's scratch, looked up once per device with that device current.
+double* scratch_on_device() {
+    int dev = 0;
+    SW_HIP(hipGetDevice(&dev));
+    SW_CHECK(dev >= 0 && dev < MAX_DEVICES, SW_ERR_ARG, "device index beyond the scratch table");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        SW_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        SW_CHECK(p != nullptr, SW_ERR_HIP, "no device address for the transform scratch");
+        g_scratch_ptr[dev] = static_cast<double*>(p);
+    }
+    return g_scratch_ptr[dev];
+}
+
 }  // namespace
 
 struct sw_d2fft {
@@ -170,6 +196,7 @@ int sw_init(int device) {
             SW_HIP(hipStreamDestroy(g_stream));
             g_stream = nullptr;
         }
+        (void)scratch_on_device();
         return 0;
     });
 }
@@ -273,6 +300,22 @@ int sw_ptospc(const sw_d2fft* d, double* rvar, double* svar) {
         Arg<double> a(rvar, e.n2, true, false, e.st), b(svar, e.n2, false, true, e.st);
         e.fwd2d(1, e.ld(a.d), e.stp(b.d));
         b.finish();
+        return 0;
+    });
+}
+
+int sw_ptospc_noalloc(const sw_d2fft* d, const double* rvar, double* svar) {
+    return guard([&] {
+        SW_CHECK(d && rvar && svar, SW_ERR_ARG, "null argument");
+        auto& e = *d->e;
+        SW_CHECK(d->nx <= NOALLOC_MAX_NG, SW_ERR_ARG,
+                 "sw_ptospc_noalloc: ng = " + std::to_string(d->nx) + " exceeds its scratch (ng <= " +
+                     std::to_string(NOALLOC_MAX_NG) + ")");
+        SW_CHECK(is_device_ptr(rvar) && is_device_ptr(svar), SW_ERR_ARG,
+                 "sw_ptospc_noalloc takes device arrays only (a host array would need a staging buffer)");
+        double* ws = scratch_on_device();
+        e.fwd2d(1, e.ld(rvar), e.stp(svar), Engine<double>::SITE_OTHER, ws);
+        SW_HIP(hipStreamSynchronize(e.st));
         return 0;
     });
 }

@@ -563,12 +563,13 @@ struct Engine {
 
     // Forward 2-D transform of nslab slabs: pro(s, x, y) gives the physical value at (y, x);
     // epi(s, ky, kx, v) receives the spectral coefficient at (kx, ky).
-    // NW: waves per block of the transposing pass (0: NWT).
+    // NW: waves per block of the transposing pass (0: NWT).  inter: the buffer between the two passes
+    // (nslab slabs; null: the engine's own, grown as needed).
     template <int NW = 0, class Pro, class Epi>
-    void fwd2d(int nslab, Pro pro, Epi epi, int site = SITE_OTHER) {
+    void fwd2d(int nslab, Pro pro, Epi epi, int site = SITE_OTHER, T* inter = nullptr) {
         if (nslab <= 0) return;
         if (wave_n()) {
-            T* t = tmp_for(nslab);
+            T* t = inter ? inter : tmp_for(nslab);
             wave_dispatch([&](auto nn) {
                 constexpr int NN = decltype(nn)::value;
                 fwd_pass1_wave<NN, NW ? NW : NWT>(nslab, pro, t, site);
@@ -576,7 +577,7 @@ struct Engine {
             });
             return;
         }
-        T* t = tmp_for(nslab);
+        T* t = inter ? inter : tmp_for(nslab);
         dim3 grid(nslab * (ng / plan.B)), block(plan.nt);
         const double bytes = 2.0 * sizeof(T) * (double)n2 * nslab;
         const bool time1 = prof == SW_PROF_PSOLVE_DERIV && site == SITE_PSOLVE_DERIV, time2 = prof == SW_PROF_STRAIGHT;
