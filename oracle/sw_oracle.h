@@ -48,6 +48,9 @@ int orc_source(void* s, double** fields, double* sqs, double* sds, double* sgs);
 int orc_diagnose(void* s, double** fields, double t, double* out5);
 int orc_advance(void* s, double** fields, double* t, int ggen, int nitmax, double* diag5, int* iters3);
 int orc_energy(void* s, double** fields, double t, double* out6);
+/* savegrid's output as numbers (src/nhswps/mod.rs:438-652): d3 = 6 (ng,ng,nz+1) f32 fields ql d g r w
+ * pn; d2 = 5 (ng,ng) f32 fields d zeta q g h; spec3 = 3 arrays of ng+1 (zeta, delta, gamma spectra) */
+int orc_savegrid(void* s, double** fields, float* d3, float* d2, double* spec3);
 int orc_run(void* s, double** fields, double* t, double tsim, double tgsave, int nitmax, double* mon, int maxmon,
             int* nmon, double* eco, int maxeco, int* neco, int* iters_total);
 #ifdef __cplusplus

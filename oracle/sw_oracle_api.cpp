@@ -199,6 +199,10 @@ int orc_energy(void* s, double** fields, double t, double* out6) {
     energy(make_state(s, fields, t, 0), out6);
     return 0;
 }
+int orc_savegrid(void* s, double** fields, float* d3, float* d2, double* spec3) {
+    savegrid(make_state(s, fields, 0.0, 0), d3, d2, spec3);
+    return 0;
+}
 int orc_run(void* s, double** fields, double* t, double tsim, double tgsave, int nitmax, double* mon, int maxmon,
             int* nmon, double* eco, int maxeco, int* neco, int* iters_total) {
     if (static_cast<Spectral*>(s)->nz < 3 || !(tgsave > 0.0)) return -1;

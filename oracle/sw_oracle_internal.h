@@ -75,6 +75,7 @@ void source(const State& st, double* sqs, double* sds, double* sgs);
 void diagnose(const State& st, double out[5]);
 int advance(State& st, int nitmax, double diag[5], int iters[3]);
 void energy(const State& st, double out[6]);
+void savegrid(const State& st, float* d3, float* d2, double* spec3);
 int run(State& st, double tsim, double tgsave, int nitmax, double* mon, int maxmon, int* nmon, double* eco, int maxeco,
         int* neco, int* iters_total);
 

@@ -777,6 +777,66 @@ void energy(const State& st, double out[6]) {
     out[5] = ekin + epot;
 }
 
+// The file output of savegrid (src/nhswps/mod.rs:438-652) as numbers: spec3 = the vertically averaged
+// 1-d spectra of zeta, delta and gamma (three arrays of ng + 1, each level's spec1d weighted by the
+// trapezoid weights, then times spmf: :438-482; spectra.asc prints k = 1..kmaxred of them as log10 with
+// alk), d3 = the 3-D fields ql, d, g (spctop of qs, ds, gs), r, w, pn as f32 (:491-573), and d2 = the
+// vertically integrated 2-D fields d, zeta, q, g, h as f32 (:575-652), in the order savegrid writes them.
+void savegrid(const State& st, float* d3, float* d2, double* spec3) {
+    const Spectral& S = *st.sp;
+    const size_t ng = S.ng, nz = S.nz, n2 = ng * ng, L = nz + 1, N = n2 * L;
+    const double *zeta = st.f[ZETA], *r = st.f[R], *z = st.f[Z], *w = st.f[W];
+    Vec wkp(n2), wks(n2), wkq(n2), tmp(ng + 1, 0.0);
+    double *zsp = spec3, *dsp = spec3 + (ng + 1), *gsp = spec3 + 2 * (ng + 1);
+    for (size_t k = 0; k < 3 * (ng + 1); k++) spec3[k] = 0.0;
+    for (size_t iz = 0; iz < L; iz++) {
+        std::memcpy(wkp.data(), zeta + n2 * iz, n2 * sizeof(double));
+        S.d2fft.ptospc(wkp.data(), wks.data());
+        S.spec1d(wks.data(), tmp.data());
+        for (size_t k = 0; k <= ng; k++) zsp[k] = zsp[k] + S.weight[iz] * tmp[k];
+        S.spec1d(st.f[DS] + n2 * iz, tmp.data());
+        for (size_t k = 0; k <= ng; k++) dsp[k] = dsp[k] + S.weight[iz] * tmp[k];
+        S.spec1d(st.f[GS] + n2 * iz, tmp.data());
+        for (size_t k = 0; k <= ng; k++) gsp[k] = gsp[k] + S.weight[iz] * tmp[k];
+    }
+    for (size_t k = 0; k <= ng; k++) {
+        zsp[k] *= S.spmf[k];
+        dsp[k] *= S.spmf[k];
+        gsp[k] *= S.spmf[k];
+    }
+    // 3-D fields: ql, d, g in physical space, then r, w, pn
+    const int spec_fields[3] = {QS, DS, GS};
+    for (int q = 0; q < 3; q++)
+        for (size_t iz = 0; iz < L; iz++) {
+            std::memcpy(wks.data(), st.f[spec_fields[q]] + n2 * iz, n2 * sizeof(double));
+            S.d2fft.spctop(wks.data(), wkp.data());
+            for (size_t p = 0; p < n2; p++) d3[q * N + n2 * iz + p] = (float)wkp[p];
+        }
+    const int phys_fields[3] = {R, W, PN};
+    for (int q = 0; q < 3; q++)
+        for (size_t e = 0; e < N; e++) d3[(3 + q) * N + e] = (float)st.f[phys_fields[q]][e];
+    // 2-D fields
+    const double* zt = z + n2 * nz;
+    const double* wt = w + n2 * nz;
+    for (size_t p = 0; p < n2; p++) d2[p] = (float)(-wt[p] / zt[p]);
+    for (size_t p = 0; p < n2; p++) wkp[p] = 0.0;
+    for (size_t iz = 0; iz < L; iz++)
+        for (size_t p = 0; p < n2; p++) wkp[p] += S.weight[iz] * zeta[n2 * iz + p] * (1.0 + r[n2 * iz + p]);
+    for (size_t p = 0; p < n2; p++) wkp[p] *= HBAR / zt[p];
+    for (size_t p = 0; p < n2; p++) d2[n2 + p] = (float)wkp[p];
+    for (size_t p = 0; p < n2; p++) wkp[p] = HBAR * (wkp[p] + COF) / zt[p] - COF;
+    for (size_t p = 0; p < n2; p++) d2[2 * n2 + p] = (float)wkp[p];
+    for (size_t p = 0; p < n2; p++) wkp[p] = 0.0;
+    for (size_t iz = 0; iz < L; iz++) {
+        std::memcpy(wks.data(), st.f[GS] + n2 * iz, n2 * sizeof(double));
+        S.d2fft.spctop(wks.data(), wkq.data());
+        for (size_t p = 0; p < n2; p++) wkp[p] += S.weight[iz] * wkq[p] * (1.0 + r[n2 * iz + p]);
+    }
+    for (size_t p = 0; p < n2; p++) wkp[p] *= HBAR / zt[p];
+    for (size_t p = 0; p < n2; p++) d2[3 * n2 + p] = (float)wkp[p];
+    for (size_t p = 0; p < n2; p++) d2[4 * n2 + p] = (float)((1.0 / HBAR) * zt[p] - 1.0);
+}
+
 // The time loop of nhswps (src/nhswps/mod.rs:285-345) from a state whose prognostic fields qs, ds, gs
 // are set: an inversion, then while t <= tsim: at save steps (itime = round(t/dt) a multiple of
 // ngsave = round(tgsave/dt)) main_invert, psolve and the savegrid energies, ggen = false, otherwise

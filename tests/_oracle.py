@@ -24,7 +24,7 @@ def lib():
         for name in ("orc_spectral_free", "orc_spectral_get", "orc_main_invert", "orc_jacob", "orc_divs",
                      "orc_ptospc3d", "orc_spctop3d", "orc_deal3d", "orc_deal2d", "orc_spec1d",
                      "orc_vertical", "orc_coeffs", "orc_cpsource", "orc_psolve", "orc_source",
-                     "orc_diagnose", "orc_advance", "orc_energy", "orc_run"):
+                     "orc_diagnose", "orc_advance", "orc_energy", "orc_run", "orc_savegrid"):
             getattr(_lib, name).argtypes = None
     return _lib
 
@@ -210,6 +210,19 @@ def energy(st):
     out = np.zeros(6)
     lib().orc_energy(st.spectral._h, st._ptrs(), C.c_double(st.t), dp(out))
     return out
+
+
+def savegrid(st):
+    """savegrid's output as numbers: (3-D f32 fields (6, N): ql d g r w pn; 2-D f32 fields (5, ng^2):
+    d zeta q g h; spectra (3, ng + 1): zeta, delta, gamma)."""
+    ng, nz = st.spectral.ng, st.spectral.nz
+    n2 = ng * ng
+    d3 = np.zeros((6, n2 * (nz + 1)), np.float32)
+    d2 = np.zeros((5, n2), np.float32)
+    spec = np.zeros((3, ng + 1))
+    lib().orc_savegrid(st.spectral._h, st._ptrs(), d3.ctypes.data_as(C.POINTER(C.c_float)),
+                       d2.ctypes.data_as(C.POINTER(C.c_float)), dp(spec))
+    return d3, d2, spec
 
 
 def run(st, tsim, tgsave, nitmax=100, max_rows=None):
