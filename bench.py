@@ -3,17 +3,20 @@
 
 A step is one full `advance` (predictor + two correctors: 3x main_invert, 3x psolve, 3x source)
 with all fields resident in HBM, on a synthetic seeded k^-3 PV state at ng=512, nz=16.  The FP64 step
-and the mixed-precision step (FP32 transforms, inversion and sources; FP64 pressure tridiagonal,
-residual and reductions) are both timed, and the mixed step's per-field relative L2 error against
-the FP64 step after one step is measured (q, delta, gamma, h, p).  The headline `value` is the
-faster mode that keeps every field within 1e-5 of the FP64 step: the mixed step if it does, else the
-FP64 step itself.  A point-step is one stored level point advanced one step: ng^2 (nz+1) per step.
+and the mixed-precision step (psolve's coefficient fields and fixed-point iteration in FP32, its
+tridiagonal and residual in FP64, every other stage and the state in FP64) are both timed, and the
+mixed step's per-field relative L2 error against the FP64 step after one step is measured (q, delta,
+gamma, h, p).  The headline `value` is the faster mode that keeps every field within 1e-5 of the FP64
+step: the mixed step if it does, else the FP64 step itself.  A point-step is one stored level point
+advanced one step: ng^2 (nz+1) per step.  The mixed mode's iteration stops at 1e-6 (the FP32
+transforms' floor), the FP64 one at the reference's 1e-9, so the FP64 step is also timed at 1e-6
+(equal tolerance).
 
-Also reported: the roofline of the step's largest kernel, the transposing FFT pass of psolve's
-theta-derivative transform (HIP events on the library stream around its launches; algorithmic
-bytes = read + write of every transformed value, 2 s ng^2 per slab), and a CPU baseline: the oracle
-(C++ restatement of the reference algorithm, OpenMP over levels) on this host's cores for one step
-of the same workload.
+Also reported: per-stage times per step and the time per psolve iteration for both modes (HIP events
+around the stages); the roofline of the largest of psolve's three kernels (HIP events on the library
+stream around its launches; algorithmic bytes per launch as documented for each); and a CPU baseline:
+the oracle (C++ restatement of the reference algorithm, OpenMP over levels, built -march=native on
+this host when a compiler is there) for one warm-up and three timed steps of the same workload.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--ng 512] [--nz 16]
 """
@@ -68,6 +71,16 @@ def time_steps(sw, spec, st, steps, warmup, prof_modes):
         iters.append(sum(it))
     sw.sync()
     dt = time.perf_counter() - t0
+    # per-stage times over two more steps
+    spec.profile(sw.PROF_STAGES)
+    for _ in range(2):
+        st.set_time(st.t, True)
+        sw.advance(st)
+    stage_ms, nsteps, nits = spec.profile_stages()
+    spec.profile(sw.PROF_OFF)
+    stages = {"ms_per_step": {k: v / nsteps for k, v in stage_ms.items()},
+              "psolve_iterations_per_step": nits / nsteps,
+              "ms_per_psolve_iteration": stage_ms["psolve_iteration"] / max(nits, 1)}
     # the same workload again, per profiled kernel, with HIP events on the library stream around each
     # launch of that kernel
     prof = {}
@@ -80,7 +93,7 @@ def time_steps(sw, spec, st, steps, warmup, prof_modes):
         ms, nbytes, launches = spec.profile_read()
         spec.profile(sw.PROF_OFF)
         prof[name] = dict(ms_per_step=ms / nstep, bytes=nbytes, launches=launches, ms=ms)
-    return dt / steps * 1e3, iters, prof
+    return dt / steps * 1e3, iters, prof, stages
 
 
 def measured_traffic(site, kernel, ng, nz, precision):
@@ -191,8 +204,8 @@ def ensemble_bench(sw, steps, warmup, nmember=64, nstreams=(1, 4)):
     """The batched-ensemble workload: nmember members, member m with ng = (64, 128, 256)[m % 3],
     nz = (4, 8)[(m // 3) % 2], k0 = (4, 8, 16)[(m // 6) % 3], seed 1000 + m, seeded k^-3 PV with rms
     zeta = 0.3 f (at f/2 the reference algorithm's pressure iteration diverges for the k0 = 4, ng = 64
-    members, and the FP32 bulk loses some k0 = 16 members), mixed precision (FP32 bulk, FP64 psolve
-    tridiagonal and residual), each member with its own spectral handle, advanced together by
+    members), mixed precision (FP32 psolve coefficient fields and iteration, FP64 elsewhere), each
+    member with its own spectral handle, advanced together by
     sw_ensemble_advance over 1 and 4 streams.  Returns the aggregate point-steps/s per stream count."""
     shapes = [((64, 128, 256)[m % 3], (4, 8)[(m // 3) % 2], (4.0, 8.0, 16.0)[(m // 6) % 3], 1000 + m)
               for m in range(nmember)]
@@ -231,19 +244,49 @@ def ensemble_bench(sw, steps, warmup, nmember=64, nstreams=(1, 4)):
     return out
 
 
-def cpu_baseline(sw, ng, nz, init):
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
+def cpu_baseline(sw, ng, nz, init, timed=3):
+    """The oracle (reference algorithm, C++ restatement) on this host: one warm-up step, then `timed`
+    steps, each from the previous (ggen set, as the GPU run).  Built -march=native on this host when
+    possible (make -C oracle native), else the prebuilt x86-64-v3 library.  Threads: the host's CPU share
+    for one GPU (16) or its core count if smaller."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import subprocess
     import _oracle
-    cores = min(os.cpu_count() or 1, 16)
-    cores = _oracle.set_threads(cores)
+    build = "-O3 -march=x86-64-v3 (prebuilt)"
+    try:
+        subprocess.run(["make", "-B", "-s", "-C", os.path.join(ROOT, "oracle"), "native"], check=True, timeout=300,
+                       stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL)
+        _oracle.LIB_PATH = os.path.join(ROOT, "oracle", "liboracle_sw_native.so")
+        build = "-O3 -march=native (built on this host)"
+    except Exception as e:  # no compiler or a failed build: time the prebuilt library
+        print(f"[bench] oracle native build unavailable ({e}); timing the prebuilt library", file=sys.stderr)
+    nproc = os.cpu_count() or 1
+    cores = _oracle.set_threads(min(nproc, 16))
     o = _oracle.State(_oracle.Spectral(ng, nz), ggen=True, **init)
-    t0 = time.perf_counter()
     rc, _, _ = _oracle.advance(o)
-    dt = time.perf_counter() - t0
     if rc != 0:
         return None
+    t0 = time.perf_counter()
+    for _ in range(timed):
+        o.ggen = True
+        rc, _, _ = _oracle.advance(o)
+        if rc != 0:
+            return None
+    dt = (time.perf_counter() - t0) / timed
     return {"value": ng * ng * (nz + 1) / dt, "unit": "point-steps/s", "cores": cores, "kind": "port",
-            "sample": f"1 advance step at ng={ng}, nz={nz} (FP64, {dt:.1f} s), same seeded state"}
+            "nproc": nproc, "cpu_model": cpu_model(), "build": build, "ms_per_step": dt * 1e3,
+            "sample": f"{timed} advance steps after 1 warm-up step at ng={ng}, nz={nz} (FP64, {dt:.1f} s per step), "
+                      f"same seeded state; reference algorithm, C++ restatement, {cores} OpenMP threads"}
 
 
 def main():
@@ -285,13 +328,15 @@ def main():
     init = initial_fields(sw, ng, nz, args.seed)
 
     res = {}
-    for p, name in ((sw.FP64, "fp64"), (sw.MIXED, "mixed")):
+    modes = {"psolve_deriv": sw.PROF_PSOLVE_DERIV, "psolve_comb": sw.PROF_PSOLVE_COMB, "psolve_tri": sw.PROF_PSOLVE_TRI}
+    for p, name, tol in ((sw.FP64, "fp64", None), (sw.MIXED, "mixed", None), (sw.FP64, "fp64_tol_1e-6", 1e-6)):
         spec, st = new_state(sw, ng, nz, p, init)
-        modes = {"psolve_deriv": sw.PROF_PSOLVE_DERIV, "psolve_comb": sw.PROF_PSOLVE_COMB, "psolve_tri": sw.PROF_PSOLVE_TRI}
-        ms, iters, prof = time_steps(sw, spec, st, args.steps, args.warmup, modes)
-        res[name] = dict(ms=ms, iters=iters, prof=prof, value=points / (ms * 1e-3))
+        if tol is not None:
+            spec.set_tolerance(tol)
+        ms, iters, prof, stages = time_steps(sw, spec, st, args.steps, args.warmup, modes if tol is None else {})
+        res[name] = dict(ms=ms, iters=iters, prof=prof, stages=stages, value=points / (ms * 1e-3))
         print(f"[bench] {name}: {ms:.3f} ms/step, {points / (ms * 1e-3):.4g} point-steps/s, psolve iterations "
-              f"per step {iters}", file=sys.stderr, flush=True)
+              f"per step {iters}; stages {stages}", file=sys.stderr, flush=True)
         del st, spec
     err = one_step_errors(sw, ng, nz, init)
     print(f"[bench] mixed vs fp64 after 1 step: {err}", file=sys.stderr, flush=True)
@@ -310,7 +355,7 @@ def main():
     # headline: the fastest precision mode that meets the accuracy bar against the FP64 step
     meets = all(v <= ACCURACY_BAR for v in err.values())
     head = "mixed" if meets else "fp64"
-    h, f64, m = res[head], res["fp64"], res["mixed"]
+    h, f64, m, f64t = res[head], res["fp64"], res["mixed"], res["fp64_tol_1e-6"]
     # roofline of the dominant profiled kernel (largest share of the step)
     site = max(h["prof"], key=lambda k: h["prof"][k]["ms_per_step"])
     pk = h["prof"][site]
@@ -343,10 +388,16 @@ def main():
         "accuracy_bar": {"rel_l2_max": ACCURACY_BAR, "mixed_meets_bar": meets,
                          "rel_l2_mixed_vs_fp64_after_1_step": err,
                          "long_runs": long_runs},
-        "fp64": {"value": f64["value"], "ms_per_step": f64["ms"], "psolve_iters_per_step": f64["iters"]},
+        "fp64": {"value": f64["value"], "ms_per_step": f64["ms"], "psolve_iters_per_step": f64["iters"],
+                 "stages": f64["stages"]},
         "mixed": {"value": m["value"], "ms_per_step": m["ms"], "psolve_iters_per_step": m["iters"],
-                  "dtype": "f32 bulk, f64 psolve tridiagonal+residual and reductions",
-                  "over_fp64": m["value"] / f64["value"]},
+                  "dtype": "f64 state and stages; f32 psolve coefficient fields and iteration (tolerance 1e-6), "
+                           "f64 tridiagonal, residual and reductions",
+                  "stages": m["stages"],
+                  "over_fp64": m["value"] / f64["value"],
+                  "over_fp64_equal_tolerance": m["value"] / f64t["value"]},
+        "fp64_tol_1e-6": {"value": f64t["value"], "ms_per_step": f64t["ms"], "psolve_iters_per_step": f64t["iters"],
+                          "stages": f64t["stages"]},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(site, kname, ng, nz, head),
                      "kernel": kname,

@@ -31,6 +31,9 @@ typedef enum {
     SW_ERR_NONFINITE = -7         /* NaN/Inf reached the psolve residual */
 } sw_status;
 
+/* SW_FP64: every stage in FP64.  SW_MIXED: psolve's coefficient fields (coeffs) and its fixed-point
+ * iteration in FP32, the pressure tridiagonal and residual in FP64, with the iteration's tolerance 1e-6
+ * (FP32 transforms put a floor near 1e-7 under the residual); the state and every other stage in FP64. */
 typedef enum { SW_FP64 = 0, SW_MIXED = 1 } sw_precision;
 
 /* Field ids of a state, in the order of the reference State (src/nhswps/mod.rs:87-133). */
@@ -131,12 +134,18 @@ int sw_ensemble_advance(sw_ensemble* e, double* diag, int* iters, int* status);
  * inverse-transform / source / forward-transform launch, SW_PROF_PSOLVE_TRI = psolve's fused forward
  * pass and column solve (the last two for ng = 128, 256, 512). */
 typedef enum {
-    SW_PROF_OFF = 0, SW_PROF_STRAIGHT = 1, SW_PROF_PSOLVE_DERIV = 2, SW_PROF_PSOLVE_COMB = 3, SW_PROF_PSOLVE_TRI = 4
+    SW_PROF_OFF = 0, SW_PROF_STRAIGHT = 1, SW_PROF_PSOLVE_DERIV = 2, SW_PROF_PSOLVE_COMB = 3, SW_PROF_PSOLVE_TRI = 4,
+    SW_PROF_STAGES = 5
 } sw_prof_mode;
 int sw_profile(sw_spectral* s, int mode);
 int sw_profile_read(sw_spectral* s, double* ms, double* bytes, long* launches);
-/* psolve stopping tolerance; default 1e-9 (psolve.rs:17) in FP64, 1e-6 in the mixed mode, where the
- * FP32 transforms put a floor under the residual. */
+/* SW_PROF_STAGES: time spent in each stage of the advance steps since sw_profile (HIP events on the
+ * library stream): ms[0] main_invert, [1] psolve before its iteration (ri, vertical, coeffs, cpsource),
+ * [2] psolve's iteration, [3] psolve after it (pn, dpn), [4] source, [5] the pointwise updates,
+ * [6] diagnose; *steps the advance steps and *iterations their psolve iterations. */
+int sw_profile_stages(sw_spectral* s, double ms[7], long* steps, long* iterations);
+/* psolve stopping tolerance; default 1e-9 (psolve.rs:17) in FP64, 1e-6 for the mixed mode's FP32
+ * iteration, where the FP32 transforms put a floor under the residual. */
 int sw_set_tolerance(sw_spectral* s, double toler);
 
 /* ---- synthetic initial state (host, no reference counterpart: the bench/test generator) ---- */

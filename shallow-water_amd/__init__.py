@@ -20,7 +20,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libsw_hip.so")
 
 FP64, MIXED = 0, 1
-PROF_OFF, PROF_STRAIGHT, PROF_PSOLVE_DERIV, PROF_PSOLVE_COMB, PROF_PSOLVE_TRI = 0, 1, 2, 3, 4
+PROF_OFF, PROF_STRAIGHT, PROF_PSOLVE_DERIV, PROF_PSOLVE_COMB, PROF_PSOLVE_TRI, PROF_STAGES = 0, 1, 2, 3, 4, 5
+STAGES = ["main_invert", "psolve_setup", "psolve_iteration", "psolve_end", "source", "update", "diagnose"]
 FIELDS = ["u", "v", "w", "z", "zx", "zy", "r", "ri", "aa", "zeta", "pn", "dpn", "ps", "qs", "ds", "gs"]
 STATUS = {0: "SW_OK", -1: "SW_ERR_ARG", -2: "SW_ERR_PSOLVE_MAXITER", -3: "SW_ERR_PSOLVE_DIVERGED",
           -4: "SW_ERR_FACTOR", -5: "SW_ERR_UNSUPPORTED", -6: "SW_ERR_HIP", -7: "SW_ERR_NONFINITE"}
@@ -163,6 +164,14 @@ class Spectral:
         ms, by, n = C.c_double(), C.c_double(), C.c_long()
         _check(_lib.sw_profile_read(self._h, C.byref(ms), C.byref(by), C.byref(n)))
         return ms.value, by.value, n.value
+
+    def profile_stages(self):
+        """After profile(PROF_STAGES) and some advance steps: (ms per stage as a dict over STAGES, steps,
+        psolve iterations)."""
+        ms = np.zeros(7)
+        steps, its = C.c_long(), C.c_long()
+        _check(_lib.sw_profile_stages(self._h, _d(ms), C.byref(steps), C.byref(its)))
+        return dict(zip(STAGES, ms.tolist())), steps.value, its.value
 
     def set_tolerance(self, toler):
         _check(_lib.sw_set_tolerance(self._h, C.c_double(toler)))

@@ -128,6 +128,8 @@ struct sw_d2fft {
     std::unique_ptr<Engine<double>> e;
 };
 
+// A spectral handle: the FP64 engine, and for SW_MIXED the FP32 engine that runs psolve's coefficient
+// fields and iteration inside it (Engine::sub).
 struct sw_spectral {
     int precision;
     size_t ng, nz;
@@ -135,19 +137,15 @@ struct sw_spectral {
     std::unique_ptr<Engine<float>> e32;
 };
 
+// The 16 fields of the reference State, FP64 in either precision mode.
 struct sw_state {
     sw_spectral* s;
     double t = 0.0;
     int ggen = 0;
     DevBuf<double> f64[SW_NFIELDS];
-    DevBuf<float> f32[SW_NFIELDS];
-    template <typename T>
-    typename Engine<T>::Fields fields() {
-        typename Engine<T>::Fields f;
-        for (int i = 0; i < SW_NFIELDS; i++) {
-            if constexpr (sizeof(T) == 8) f.a[i] = f64[i].p;
-            else f.a[i] = f32[i].p;
-        }
+    Engine<double>::Fields fields() {
+        Engine<double>::Fields f;
+        for (int i = 0; i < SW_NFIELDS; i++) f.a[i] = f64[i].p;
         return f;
     }
 };
@@ -163,21 +161,17 @@ struct sw_ensemble {
     }
 };
 
-// Run fn(engine) with the engine of the spectral handle's precision.
+// Run fn(engine) with the handle's engine (the FP64 one; in SW_MIXED it carries the FP32 one).
 template <class F>
 static int with_engine(sw_spectral* s, F fn) {
     SW_CHECK(s != nullptr, SW_ERR_ARG, "null spectral handle");
-    if (s->precision == SW_FP64) return fn(*s->e64);
-    return fn(*s->e32);
+    return fn(*s->e64);
 }
 
 template <class F>
 static int with_state(sw_state* st, F fn) {
     SW_CHECK(st && st->s, SW_ERR_ARG, "null state");
-    return with_engine(st->s, [&](auto& e) {
-        using T = std::remove_reference_t<decltype(*e.tmp.p)>;
-        return fn(e, st->template fields<T>());
-    });
+    return with_engine(st->s, [&](auto& e) { return fn(e, st->fields()); });
 }
 
 extern "C" {
@@ -365,8 +359,11 @@ int sw_spectral_create(size_t ng, size_t nz, int precision, sw_spectral** out) {
         s->precision = precision;
         s->ng = ng;
         s->nz = nz;
-        if (precision == SW_FP64) s->e64 = std::make_unique<Engine<double>>((int)ng, (int)nz, stream());
-        else s->e32 = std::make_unique<Engine<float>>((int)ng, (int)nz, stream());
+        s->e64 = std::make_unique<Engine<double>>((int)ng, (int)nz, stream());
+        if (precision == SW_MIXED) {
+            s->e32 = std::make_unique<Engine<float>>((int)ng, (int)nz, stream());
+            s->e64->sub = s->e32.get();
+        }
         *out = s.release();
         return 0;
     });
@@ -380,7 +377,7 @@ int sw_spectral_destroy(sw_spectral* s) {
 long sw_spectral_get(const sw_spectral* s, const char* name, double* out) {
     try {
         SW_CHECK(s && name, SW_ERR_ARG, "null argument");
-        const HostOps& h = s->precision == SW_FP64 ? s->e64->host : s->e32->host;
+        const HostOps& h = s->e64->host;
         std::string n(name);
         const std::vector<double>* v = nullptr;
         if (n == "hlap") v = &h.hlap; else if (n == "glap") v = &h.glap; else if (n == "rlap") v = &h.rlap;
@@ -487,13 +484,8 @@ int sw_state_create(sw_spectral* s, sw_state** out) {
         size_t N = s->ng * s->ng * (s->nz + 1);
         hipStream_t hs = stream();
         for (int i = 0; i < SW_NFIELDS; i++) {
-            if (s->precision == SW_FP64) {
-                st->f64[i].alloc(N);
-                SW_HIP(hipMemsetAsync(st->f64[i].p, 0, N * sizeof(double), hs));
-            } else {
-                st->f32[i].alloc(N);
-                SW_HIP(hipMemsetAsync(st->f32[i].p, 0, N * sizeof(float), hs));
-            }
+            st->f64[i].alloc(N);
+            SW_HIP(hipMemsetAsync(st->f64[i].p, 0, N * sizeof(double), hs));
         }
         SW_HIP(hipStreamSynchronize(hs));
         *out = st.release();
@@ -730,17 +722,17 @@ int sw_ensemble_advance(sw_ensemble* e, double* diag, int* iters, int* status) {
                 try {
                     rc[i] = with_state(st, [&](auto& eng, auto f) {
                         hipStream_t old = eng.st;
-                        eng.st = e->streams[g];
+                        eng.set_stream(e->streams[g]);
                         int r = 0;
                         try {
                             r = eng.advance(f, st->t, st->ggen != 0, &dg[(size_t)5 * i], &it[(size_t)3 * i]);
                             SW_HIP(hipStreamSynchronize(eng.st));
                         } catch (...) {
                             (void)hipStreamSynchronize(eng.st);
-                            eng.st = old;
+                            eng.set_stream(old);
                             throw;
                         }
-                        eng.st = old;
+                        eng.set_stream(old);
                         return r;
                     });
                 } catch (const Error& x) {
@@ -770,12 +762,19 @@ int sw_ensemble_advance(sw_ensemble* e, double* diag, int* iters, int* status) {
 
 int sw_profile(sw_spectral* s, int mode) {
     return guard([&] {
-        SW_CHECK(mode >= SW_PROF_OFF && mode <= SW_PROF_PSOLVE_TRI, SW_ERR_ARG, "unknown profiling mode");
+        SW_CHECK(mode >= SW_PROF_OFF && mode <= SW_PROF_STAGES, SW_ERR_ARG, "unknown profiling mode");
         return with_engine(s, [&](auto& e) {
-            e.prof_collect();
-            e.prof = mode;
-            e.prof_ms = e.prof_bytes = 0.0;
-            e.prof_launches = 0;
+            auto reset = [&](auto& x) {
+                x.prof_collect();
+                x.stage_collect();
+                x.prof = mode;
+                x.prof_ms = x.prof_bytes = 0.0;
+                x.prof_launches = 0;
+                for (double& m : x.stage_ms) m = 0.0;
+                x.stage_steps = x.stage_iters = 0;
+            };
+            reset(e);
+            if (e.sub) reset(*e.sub);
             return 0;
         });
     });
@@ -789,6 +788,26 @@ int sw_profile_read(sw_spectral* s, double* ms, double* bytes, long* launches) {
             *ms = e.prof_ms;
             *bytes = e.prof_bytes;
             *launches = e.prof_launches;
+            if (e.sub) {
+                e.sub->prof_collect();
+                *ms += e.sub->prof_ms;
+                *bytes += e.sub->prof_bytes;
+                *launches += e.sub->prof_launches;
+            }
+            return 0;
+        });
+    });
+}
+
+int sw_profile_stages(sw_spectral* s, double ms[7], long* steps, long* iterations) {
+    return guard([&] {
+        SW_CHECK(ms && steps && iterations, SW_ERR_ARG, "null argument");
+        return with_engine(s, [&](auto& e) {
+            e.stage_collect();
+            static_assert(std::remove_reference_t<decltype(e)>::NSTAGE == 7, "seven stages");
+            for (int k = 0; k < 7; k++) ms[k] = e.stage_ms[k];
+            *steps = e.stage_steps;
+            *iterations = e.stage_iters;
             return 0;
         });
     });
@@ -798,7 +817,8 @@ int sw_set_tolerance(sw_spectral* s, double toler) {
     return guard([&] {
         SW_CHECK(toler > 0.0, SW_ERR_ARG, "tolerance must be positive");
         return with_engine(s, [&](auto& e) {
-            e.toler = toler;
+            if (e.sub) e.sub->toler = toler;  // the mixed mode's iteration runs in the FP32 engine
+            else e.toler = toler;
             return 0;
         });
     });

@@ -399,8 +399,50 @@ struct Engine {
         }
         ev_used = 0;
     }
+    // stage timing (SW_PROF_STAGES): HIP events on this engine's stream around the stages of advance;
+    // stage(k) closes the open stage and opens k (k < 0: closes only)
+    enum { ST_INVERT = 0, ST_PSETUP, ST_PITER, ST_PEND, ST_SOURCE, ST_UPDATE, ST_DIAG, NSTAGE };
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> sev;
+    std::vector<int> sev_kind;
+    size_t sev_used = 0;
+    int stage_open = -1;
+    double stage_ms[NSTAGE] = {};
+    long stage_iters = 0, stage_steps = 0;
+    void stage(int k) {
+        if (prof != SW_PROF_STAGES) return;
+        if (stage_open >= 0) {
+            SW_HIP(hipEventRecord(sev[sev_used].second, st));
+            sev_used++;
+        }
+        stage_open = k;
+        if (k < 0) return;
+        if (sev_used == sev.size()) {
+            hipEvent_t a, b;
+            SW_HIP(hipEventCreate(&a));
+            SW_HIP(hipEventCreate(&b));
+            sev.push_back({a, b});
+            sev_kind.push_back(0);
+        }
+        sev_kind[sev_used] = k;
+        SW_HIP(hipEventRecord(sev[sev_used].first, st));
+    }
+    void stage_collect() {
+        stage(-1);
+        if (!sev_used) return;
+        SW_HIP(hipStreamSynchronize(st));
+        for (size_t i = 0; i < sev_used; i++) {
+            float ms = 0.f;
+            SW_HIP(hipEventElapsedTime(&ms, sev[i].first, sev[i].second));
+            stage_ms[sev_kind[i]] += ms;
+        }
+        sev_used = 0;
+    }
     ~Engine() {
         for (auto& e : ev) {
+            (void)hipEventDestroy(e.first);
+            (void)hipEventDestroy(e.second);
+        }
+        for (auto& e : sev) {
             (void)hipEventDestroy(e.first);
             (void)hipEventDestroy(e.second);
         }
@@ -817,6 +859,24 @@ struct Engine {
         }, stp(cs));
     }
 
+    // ---------------------------------------------------------------- mixed precision
+    // SW_MIXED: the FP32 engine (same ng, nz and stream) that computes psolve's coefficient fields and
+    // runs its iteration; null in FP64.  Every other stage, and the state, stay in FP64: the FP32
+    // rounding of a physical field is white in wavenumber, and the Laplacians and derivative products
+    // downstream of the inversion, vertical, cpsource and source amplify it by up to (ng/3)^2, whereas
+    // the pressure iteration solves an elliptic problem, which damps it.
+    Engine<float>* sub = nullptr;
+    void to_f32(const T* a, float* b, size_t n) {
+        each(st, n, [=] __device__(size_t i) { b[i] = (float)a[i]; });
+    }
+    void to_f64(const float* a, T* b, size_t n) {
+        each(st, n, [=] __device__(size_t i) { b[i] = (T)a[i]; });
+    }
+    void set_stream(hipStream_t s) {
+        st = s;
+        if (sub) sub->st = s;
+    }
+
     // ---------------------------------------------------------------- nhswps and the time loop
     // Defined in sw_engine_ops.inc and instantiated once per precision (sw_engine_f64.hip,
     // sw_engine_f32.hip).
@@ -831,6 +891,8 @@ struct Engine {
     bool psolve_solve_wave(const T* SPt, T* SP, T* ps, double* sums);
     void psolve_iterate_device(const T* sp0, const T* sigx, const T* sigy, const T* cpt1, const T* cpt2, T* SP, T* ps,
                                const T* pn, int nitmax, int* iters, double* errp, int* status);
+    int iterate(const T* sp0, const T* sigx, const T* sigy, const T* cpt1, const T* cpt2, T* SP, T* ps, const T* pn,
+                int nitmax, int* iters_out, double* errp_out);
     int psolve(Fields f, int nitmax, int* iters_out, double* errp_out);
     void source(Fields f, T* sqs, T* sds, T* sgs, bool reuse_derivs = false);
     void diagnose(Fields f, double t, double out[5]);
