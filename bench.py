@@ -302,6 +302,9 @@ def main():
                     help="timed steps of the 64-member ensemble workload reported beside the headline (0: skip)")
     ap.add_argument("--long-steps", type=int, default=1000,
                     help="steps of the mixed-vs-FP64 accuracy run reported beside the one-step errors (0: skip)")
+    ap.add_argument("--only", choices=["fp64", "mixed"], default=None,
+                    help="only advance the state --steps steps in this precision after --warmup steps, with no "
+                         "report (the run that rocprofv3 profiles)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (twice) and exit: the reference reading "
                          "for the FETCH_SIZE/WRITE_SIZE counters")
@@ -326,6 +329,14 @@ def main():
     ng, nz = args.ng, args.nz
     points = ng * ng * (nz + 1)
     init = initial_fields(sw, ng, nz, args.seed)
+    if args.only:
+        spec, st = new_state(sw, ng, nz, sw.FP64 if args.only == "fp64" else sw.MIXED, init)
+        for k in range(args.warmup + args.steps):
+            st.set_time(st.t, True)
+            _, it = sw.advance(st)
+            print(f"[bench] {args.only} step {k}: psolve iterations {it}", file=sys.stderr, flush=True)
+        sw.sync()
+        return
 
     res = {}
     modes = {"psolve_deriv": sw.PROF_PSOLVE_DERIV, "psolve_comb": sw.PROF_PSOLVE_COMB, "psolve_tri": sw.PROF_PSOLVE_TRI}

@@ -114,6 +114,27 @@ int sw_energy(sw_state* st, double out[6]);
 int sw_run(sw_state* st, double tsim, double tgsave, double* monitor, int max_monitor, int* nmonitor, double* ecomp,
            int max_ecomp, int* necomp, int* iters_total);
 
+/* ---- on-disk formats and the whole program: src/nhswps/mod.rs ---- */
+/* n little-endian f64 values after an 8-byte header (the *_init.r8 files the loader reads, mod.rs:159-188);
+ * SW_ERR_ARG when the file is not exactly that size. */
+int sw_read_r8(const char* path, double* out, size_t n);
+/* savegrid's output (mod.rs:347-652) as numbers, into host arrays: d3 = the 3-D fields ql, d, g, r, w, pn
+ * as f32 (6 x ng*ng*(nz+1)), d2 = the vertically integrated d, zeta, q, g, h as f32 (5 x ng*ng), spec3 =
+ * the zeta, delta and gamma spectra (3 x (ng+1); spectra.asc prints log10 of entries 1..ng/2).  Its
+ * energies are sw_energy. */
+int sw_savegrid(sw_state* st, float* d3, float* d2, double* spec3);
+/* nhswps(&Parameters) (mod.rs:150-345): reads dir/{qq,dd,gg}_init.r8 (ng*ng*(nz+1) values each), forms the
+ * spectral state as the loader does, runs the time loop to t > duration with saves every save_interval
+ * (the cadence counted in time_step units; the solver's step is 1/ng), and writes dir/monitor.asc,
+ * dir/ecomp.asc, dir/spectra.asc, dir/3d/{ql,d,g,r,w,pn}.r4 and dir/2d/{d,zeta,q,g,h}.r4 in the reference's
+ * formats.  *iters_total = the psolve iterations of the run. */
+int sw_nhswps(const char* dir, size_t ng, size_t nz, double time_step, double duration, double save_interval,
+              int precision, int* iters_total);
+/* FP64 checkpoint of a state in the r8 layout: an 8-byte header holding t, then the 16 fields in the order
+ * of the State (no reference counterpart: the reference keeps only the t = 0 *_init.r8 files). */
+int sw_state_save(sw_state* st, const char* path);
+int sw_state_load(sw_state* st, const char* path);
+
 /* ---- ensemble: members advanced together (no reference function; the ensemble configuration of
  * the batched workload) ---- */
 typedef struct sw_ensemble sw_ensemble;

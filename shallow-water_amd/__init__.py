@@ -234,6 +234,13 @@ class State:
         _check(_lib.sw_state_get_time(self._h, C.byref(t)))
         return t.value
 
+    def save(self, path):
+        """FP64 checkpoint: an 8-byte header holding t, then the 16 fields in State order."""
+        _check(_lib.sw_state_save(self._h, os.fsencode(path)))
+
+    def load(self, path):
+        _check(_lib.sw_state_load(self._h, os.fsencode(path)))
+
     def __getattr__(self, name):
         if name in FIELDS:
             return self.get(name)
@@ -296,6 +303,36 @@ def run(st, tsim, tgsave, max_rows=None):
     _check(_lib.sw_run(st._h, C.c_double(tsim), C.c_double(tgsave), _d(mon), C.c_int(max_rows), C.byref(nm), _d(eco),
                        C.c_int(max_rows), C.byref(ne), C.byref(it)))
     return mon[:min(nm.value, max_rows)], eco[:min(ne.value, max_rows)], it.value
+
+
+def savegrid(st):
+    """savegrid's output (src/nhswps/mod.rs:347-652): (3-D f32 fields (6, N): ql d g r w pn; 2-D f32 fields
+    (5, ng^2): d zeta q g h; spectra (3, ng + 1): zeta, delta, gamma)."""
+    ng, nz = st.spectral.ng, st.spectral.nz
+    n2 = ng * ng
+    d3 = np.zeros((6, n2 * (nz + 1)), np.float32)
+    d2 = np.zeros((5, n2), np.float32)
+    spec = np.zeros((3, ng + 1))
+    _check(_lib.sw_savegrid(st._h, d3.ctypes.data_as(C.POINTER(C.c_float)), d2.ctypes.data_as(C.POINTER(C.c_float)),
+                            _d(spec)))
+    return d3, d2, spec
+
+
+def read_r8(path, n):
+    """n little-endian f64 values after an 8-byte header (the *_init.r8 layout)."""
+    out = np.zeros(n)
+    _check(_lib.sw_read_r8(os.fsencode(path), _d(out), C.c_size_t(n)))
+    return out
+
+
+def nhswps(out_dir, ng, nz, time_step, duration, save_interval, precision=FP64):
+    """nhswps(&Parameters) (src/nhswps/mod.rs:150-345) on the GPU: reads out_dir/{qq,dd,gg}_init.r8, runs
+    the time loop and writes monitor.asc, ecomp.asc, spectra.asc, 2d/*.r4 and 3d/*.r4 into out_dir.
+    Returns the total psolve iterations."""
+    it = C.c_int()
+    _check(_lib.sw_nhswps(os.fsencode(out_dir), C.c_size_t(ng), C.c_size_t(nz), C.c_double(time_step),
+                          C.c_double(duration), C.c_double(save_interval), C.c_int(precision), C.byref(it)))
+    return it.value
 
 
 class Ensemble:

@@ -3,8 +3,11 @@
 // precision of the handle and turns exceptions into sw_status codes.
 #include <hip/hip_runtime.h>
 
+#include <sys/stat.h>
+
 #include <cmath>
 #include <cstdint>
+#include <cstdio>
 #include <cstring>
 #include <memory>
 #include <mutex>
@@ -645,13 +648,190 @@ int sw_run(sw_state* st, double tsim, double tgsave, double* monitor, int max_mo
         SW_CHECK((monitor || max_monitor == 0) && (ecomp || max_ecomp == 0), SW_ERR_ARG, "null row buffer");
         int nm = 0, ne = 0, itot = 0;
         int rc = with_state(st, [&](auto& e, auto f) {
-            int r = e.run(f, st->t, tsim, tgsave, monitor, max_monitor, &nm, ecomp, max_ecomp, &ne, &itot);
+            int r = e.run(f, st->t, tsim, tgsave, 1.0 / (double)st->s->ng, monitor, max_monitor, &nm, ecomp, max_ecomp,
+                          &ne, &itot);
             SW_HIP(hipStreamSynchronize(e.st));
             return r;
         });
         st->ggen = 0;
         if (nmonitor) *nmonitor = nm;
         if (necomp) *necomp = ne;
+        if (iters_total) *iters_total = itot;
+        return rc;
+    });
+}
+
+// ---------------------------------------------------------------- on-disk formats (src/nhswps/mod.rs)
+namespace {
+
+struct File {
+    FILE* f = nullptr;
+    File(const std::string& path, const char* mode) : f(std::fopen(path.c_str(), mode)) {
+        SW_CHECK(f != nullptr, SW_ERR_ARG, "cannot open " + path);
+    }
+    ~File() {
+        if (f) std::fclose(f);
+    }
+    File(const File&) = delete;
+    File& operator=(const File&) = delete;
+    void write(const void* p, size_t bytes) {
+        SW_CHECK(std::fwrite(p, 1, bytes, f) == bytes, SW_ERR_ARG, "short write");
+    }
+};
+
+// n little-endian f64 values after an 8-byte header: the *_init.r8 records of the nhswps loader
+// (mod.rs:159-188) and the checkpoint's layout.  The host is little-endian (x86-64).
+void read_r8(const std::string& path, double* out, size_t n, double* header = nullptr) {
+    File f(path, "rb");
+    SW_CHECK(std::fseek(f.f, 0, SEEK_END) == 0, SW_ERR_ARG, "cannot seek " + path);
+    const long size = std::ftell(f.f);
+    SW_CHECK(size == (long)(8 + 8 * n), SW_ERR_ARG,
+             path + ": " + std::to_string(size) + " bytes, expected an 8-byte header and " + std::to_string(n) +
+                 " f64 values (" + std::to_string(8 + 8 * n) + " bytes)");
+    std::rewind(f.f);
+    double h = 0.0;
+    SW_CHECK(std::fread(&h, 8, 1, f.f) == 1 && std::fread(out, 8, n, f.f) == n, SW_ERR_ARG, "short read of " + path);
+    if (header) *header = h;
+}
+
+// append_output (mod.rs:654-663): a record of t then the values, all little-endian f32
+void append_r4(File& f, double t, const float* data, size_t n) {
+    const float tf = (float)t;
+    f.write(&tf, 4);
+    f.write(data, 4 * n);
+}
+
+}  // namespace
+
+int sw_read_r8(const char* path, double* out, size_t n) {
+    return guard([&] {
+        SW_CHECK(path && out, SW_ERR_ARG, "null argument");
+        read_r8(path, out, n);
+        return 0;
+    });
+}
+
+int sw_savegrid(sw_state* st, float* d3, float* d2, double* spec3) {
+    return guard([&] {
+        SW_CHECK(d3 && d2 && spec3, SW_ERR_ARG, "null output");
+        return with_state(st, [&](auto& e, auto f) {
+            e.savegrid(f, d3, d2, spec3);
+            return 0;
+        });
+    });
+}
+
+int sw_state_save(sw_state* st, const char* path) {
+    return guard([&] {
+        SW_CHECK(st && st->s && path, SW_ERR_ARG, "null argument");
+        const size_t N = st->s->ng * st->s->ng * (st->s->nz + 1);
+        std::vector<double> h(N);
+        File f(path, "wb");
+        f.write(&st->t, 8);
+        return with_state(st, [&](auto& e, auto fl) {
+            for (int k = 0; k < SW_NFIELDS; k++) {
+                copy_d2h(h.data(), fl.a[k], N * sizeof(double), e.st);
+                f.write(h.data(), N * sizeof(double));
+            }
+            return 0;
+        });
+    });
+}
+
+int sw_state_load(sw_state* st, const char* path) {
+    return guard([&] {
+        SW_CHECK(st && st->s && path, SW_ERR_ARG, "null argument");
+        const size_t N = st->s->ng * st->s->ng * (st->s->nz + 1);
+        std::vector<double> h((size_t)SW_NFIELDS * N);
+        double t = 0.0;
+        read_r8(path, h.data(), h.size(), &t);
+        return with_state(st, [&](auto& e, auto fl) {
+            for (int k = 0; k < SW_NFIELDS; k++) copy_h2d(fl.a[k], h.data() + (size_t)k * N, N * sizeof(double), e.st);
+            st->t = t;
+            return 0;
+        });
+    });
+}
+
+int sw_nhswps(const char* dir, size_t ng, size_t nz, double time_step, double duration, double save_interval, int precision,
+              int* iters_total) {
+    return guard([&] {
+        SW_CHECK(dir && time_step > 0.0 && save_interval > 0.0 && std::round(save_interval / time_step) >= 1.0, SW_ERR_ARG,
+                 "bad nhswps parameters");
+        const std::string d(dir);
+        const size_t n2 = ng * ng, N = n2 * (nz + 1);
+        std::vector<double> init[3];
+        const char* names[3] = {"qq_init.r8", "dd_init.r8", "gg_init.r8"};
+        for (int k = 0; k < 3; k++) {
+            init[k].resize(N);
+            read_r8(d + "/" + names[k], init[k].data(), N);
+        }
+        sw_spectral* sp = nullptr;
+        int rc = sw_spectral_create(ng, nz, precision, &sp);
+        if (rc) throw Error(rc, g_err);
+        std::unique_ptr<sw_spectral, int (*)(sw_spectral*)> sp_guard(sp, sw_spectral_destroy);
+        sw_state* stp = nullptr;
+        rc = sw_state_create(sp, &stp);
+        if (rc) throw Error(rc, g_err);
+        std::unique_ptr<sw_state, int (*)(sw_state*)> st_guard(stp, sw_state_destroy);
+        auto& e = *sp->e64;
+        auto f = stp->fields();
+        // the loader (mod.rs:238-275): to spectral space, the (0, 0) mode zeroed, times the filter
+        const int fid[3] = {SW_QS, SW_DS, SW_GS};
+        for (int k = 0; k < 3; k++) {
+            Arg<double> a(init[k].data(), N, true, false, e.st);
+            e.ptospc3d(a.d, f.a[fid[k]], 0, (int)nz);
+            double* x = f.a[fid[k]];
+            const double* F = e.filt.p;
+            const size_t nn = n2;
+            each(e.st, N, [=] __device__(size_t q) {
+                const size_t p = q % nn;
+                x[q] = p == 0 ? 0.0 : x[q] * F[p];
+            });
+            SW_HIP(hipStreamSynchronize(e.st));
+        }
+        mkdir((d + "/2d").c_str(), 0777);
+        mkdir((d + "/3d").c_str(), 0777);
+        struct Out : Engine<double>::RunHooks {
+            Engine<double>* e = nullptr;
+            Engine<double>::Fields f{};
+            double* t = nullptr;
+            size_t n2 = 0, N = 0;
+            File mon, eco, spc, ql, d3, g3, r3, w3, p3, q2, d2, g2, h2, z2;
+            explicit Out(const std::string& d)
+                : mon(d + "/monitor.asc", "w"), eco(d + "/ecomp.asc", "w"), spc(d + "/spectra.asc", "w"),
+                  ql(d + "/3d/ql.r4", "wb"), d3(d + "/3d/d.r4", "wb"), g3(d + "/3d/g.r4", "wb"), r3(d + "/3d/r.r4", "wb"),
+                  w3(d + "/3d/w.r4", "wb"), p3(d + "/3d/pn.r4", "wb"), q2(d + "/2d/q.r4", "wb"), d2(d + "/2d/d.r4", "wb"),
+                  g2(d + "/2d/g.r4", "wb"), h2(d + "/2d/h.r4", "wb"), z2(d + "/2d/zeta.r4", "wb") {}
+            // diagnose's line (mod.rs:700-708)
+            void monitor(const double r[5]) override {
+                std::fprintf(mon.f, "%.5f %.6f %.6f %.6f %.6f\n", r[0], r[1], r[2], r[3], r[4]);
+            }
+            // savegrid (mod.rs:416-420, 474-489, 491-652)
+            void save(const double en[6]) override {
+                std::fprintf(eco.f, "%.6f %.9f %.9f %.9f %.9f %.9f\n", en[0], en[1], en[2], en[3], en[4], en[5]);
+                std::vector<float> f3(6 * N), f2(5 * n2);
+                std::vector<double> sp(3 * (e->ng + 1));
+                e->savegrid(f, f3.data(), f2.data(), sp.data());
+                const size_t nk = e->ng + 1, kr = e->host.kmaxred;
+                std::fprintf(spc.f, "%.6f %zu\n", *t, kr);
+                for (size_t k = 1; k <= kr; k++)
+                    std::fprintf(spc.f, "%.8f %.8f %.8f %.8f\n", e->host.alk[k - 1], std::log10(sp[k]),
+                                 std::log10(sp[nk + k] + 1.0E-32), std::log10(sp[2 * nk + k] + 1.0E-32));
+                File* o3[6] = {&ql, &d3, &g3, &r3, &w3, &p3};
+                for (int q = 0; q < 6; q++) append_r4(*o3[q], *t, f3.data() + (size_t)q * N, N);
+                File* o2[5] = {&d2, &z2, &q2, &g2, &h2};
+                for (int q = 0; q < 5; q++) append_r4(*o2[q], *t, f2.data() + (size_t)q * n2, n2);
+            }
+        } out(d);
+        out.e = &e;
+        out.f = f;
+        out.t = &stp->t;
+        out.n2 = n2;
+        out.N = N;
+        int nm = 0, ne = 0, itot = 0;
+        rc = e.run(f, stp->t, duration, save_interval, time_step, nullptr, 0, &nm, nullptr, 0, &ne, &itot, &out);
+        SW_HIP(hipStreamSynchronize(e.st));
         if (iters_total) *iters_total = itot;
         return rc;
     });

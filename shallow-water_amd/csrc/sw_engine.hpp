@@ -109,6 +109,21 @@ __global__ void __launch_bounds__(256) k_reduce(size_t n, F f, double* part) {
     }
 }
 
+// spec1d (spectral/mod.rs:558-591) of nslab spectral slabs: block (k, s) sums w v^2 over the members
+// of shell k of slab s (a wave, fixed order: deterministic) into out[s (ng + 1) + k].
+static __global__ void __launch_bounds__(64) k_shell_sums(const double* x, size_t nn, const int* idx, const int* off,
+                                                   const double* w, int ng, double* out) {
+    const int k = blockIdx.x, s = blockIdx.y;
+    const double* a = x + (size_t)s * nn;
+    double acc = 0.0;
+    for (int i = off[k] + (int)threadIdx.x; i < off[k + 1]; i += 64) {
+        const double v = a[idx[i]];
+        acc += w[i] * (v * v);
+    }
+    for (int o = 32; o > 0; o >>= 1) acc += __shfl_down(acc, o, 64);
+    if (threadIdx.x == 0) out[(size_t)s * (ng + 1) + k] = acc;
+}
+
 // Per-level partial sums of two products over the points of each level: grid (nb, nlev).
 template <class F>
 __global__ void __launch_bounds__(256) k_level_sum2(size_t n2, F f, double* part) {
@@ -896,9 +911,24 @@ struct Engine {
     int psolve(Fields f, int nitmax, int* iters_out, double* errp_out);
     void source(Fields f, T* sqs, T* sds, T* sgs, bool reuse_derivs = false);
     void diagnose(Fields f, double t, double out[5]);
+    // savegrid's output (src/nhswps/mod.rs:438-652): d3 = ql, d, g, r, w, pn as f32 (6 fields), d2 = the
+    // vertically integrated d, zeta, q, g, h as f32 (5 slabs), spec3 = the zeta, delta, gamma spectra
+    // (3 x (ng + 1)); host arrays
+    void savegrid(Fields f, float* d3, float* d2, double* spec3);
+    // spec1d's shells (spectral/mod.rs:558-591): the packed indices of each shell k = kmag in the
+    // order spec1d visits them, with its weights (1/4 at (0, 0), 1/2 on the axes, 1 elsewhere)
+    DevBuf<int> shell_idx, shell_off;
+    DevBuf<double> shell_w;
     void energy(Fields f, double t, double out[6]);
-    int run(Fields f, double& t, double tsim, double tgsave, double* mon, int maxmon, int* nmon, double* eco, int maxeco,
-            int* neco, int* iters_total);
+    // observer of the time loop: a monitor row per step, and each save (after its energies, with the
+    // state's fields inverted and the pressure solved)
+    struct RunHooks {
+        virtual ~RunHooks() = default;
+        virtual void monitor(const double d[5]) { (void)d; }
+        virtual void save(const double e[6]) { (void)e; }
+    };
+    int run(Fields f, double& t, double tsim, double tgsave, double tstep, double* mon, int maxmon, int* nmon,
+            double* eco, int maxeco, int* neco, int* iters_total, RunHooks* hooks = nullptr);
     int advance(Fields f, double& t, bool ggen, double diag[5], int iters[3]);
 };
 

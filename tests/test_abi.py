@@ -60,3 +60,15 @@ def test_synthetic_field_is_seeded_and_scaled(sw, oracle):
     r, u, v, zeta = s.field(), s.field(), s.field(), s.field()
     s.main_invert(qs, ds, ds.copy(), r, u, v, zeta)
     assert abs(np.sqrt(np.mean(zeta ** 2)) - 2 * np.pi) < 1e-9
+
+
+def test_read_r8_reads_values_after_the_header_and_checks_the_size(sw, tmp_path):
+    vals = np.arange(12, dtype="<f8") * 0.25 - 1.0
+    p = tmp_path / "x_init.r8"
+    p.write_bytes(np.zeros(1, "<f8").tobytes() + vals.tobytes())
+    assert np.array_equal(sw.read_r8(str(p), 12), vals)
+    with pytest.raises(sw.SwError) as e:
+        sw.read_r8(str(p), 13)
+    assert e.value.code == -1 and "expected an 8-byte header" in str(e.value)
+    with pytest.raises(sw.SwError):
+        sw.read_r8(str(tmp_path / "missing.r8"), 1)
