@@ -200,19 +200,20 @@ def long_run_errors(sw, ng, nz, init, steps):
             "fp64_monitor_t_zrms2half_zrms_zmax_umax": diag}
 
 
-def ensemble_bench(sw, steps, warmup, nmember=64, nstreams=(1, 4)):
+def ensemble_bench(sw, steps, warmup, nmember=64, runs=((4, False), (4, True), (1, False), (1, True))):
     """The batched-ensemble workload: nmember members, member m with ng = (64, 128, 256)[m % 3],
     nz = (4, 8)[(m // 3) % 2], k0 = (4, 8, 16)[(m // 6) % 3], seed 1000 + m, seeded k^-3 PV with rms
     zeta = 0.3 f (at f/2 the reference algorithm's pressure iteration diverges for the k0 = 4, ng = 64
     members), mixed precision (FP32 psolve coefficient fields and iteration, FP64 elsewhere), each
     member with its own spectral handle, advanced together by
-    sw_ensemble_advance over 1 and 4 streams.  Returns the aggregate point-steps/s per stream count."""
+    sw_ensemble_advance on 1 or 4 streams, each member's step replayed from a recorded hipGraph or launched
+    eagerly.  Returns the aggregate point-steps/s per run."""
     shapes = [((64, 128, 256)[m % 3], (4, 8)[(m // 3) % 2], (4.0, 8.0, 16.0)[(m // 6) % 3], 1000 + m)
               for m in range(nmember)]
     points = sum(ng * ng * (nz + 1) for ng, nz, _, _ in shapes)
     out = {"members": nmember, "points_per_step": points, "precision": "mixed", "rms_zeta": "0.3 f",
            "shapes": "ng=(64,128,256)[m%3], nz=(4,8)[(m//3)%2], k0=(4,8,16)[(m//6)%3], seed=1000+m"}
-    for ns in nstreams:
+    for ns, graphs in runs:
         states = []
         for ng, nz, k0, seed in shapes:
             qs, ds = sw.synth_kminus3(ng, nz, seed=seed, k0=k0, zeta_rms=0.3 * 4 * np.pi)
@@ -224,6 +225,7 @@ def ensemble_bench(sw, steps, warmup, nmember=64, nstreams=(1, 4)):
             states.append(sw.State(sw.Spectral(ng, nz, sw.MIXED), ggen=True, qs=qs, ds=ds, gs=gs, r=r, u=u, v=v,
                                    zeta=zeta))
         ens = sw.Ensemble(states, nstreams=ns)
+        ens.set_graphs(graphs)
 
         def step():
             for st in states:
@@ -233,13 +235,17 @@ def ensemble_bench(sw, steps, warmup, nmember=64, nstreams=(1, 4)):
         for _ in range(warmup):
             step()
         sw.sync()
+        r0 = ens.stats()
         t0 = time.perf_counter()
         for _ in range(steps):
             step()
         dt = (time.perf_counter() - t0) / steps
-        out[f"streams_{ns}"] = {"ms_per_step": dt * 1e3, "point_steps_per_s": points / dt}
-        print(f"[bench] ensemble of {nmember} on {ns} stream(s): {dt * 1e3:.2f} ms/step, {points / dt:.4g} point-steps/s",
-              file=sys.stderr, flush=True)
+        r1 = ens.stats()
+        key = f"streams_{ns}_" + ("graphs" if graphs else "eager")
+        out[key] = {"ms_per_step": dt * 1e3, "point_steps_per_s": points / dt,
+                    "replays": r1[0] - r0[0], "redos": r1[1] - r0[1], "captures": r1[2] - r0[2]}
+        print(f"[bench] ensemble of {nmember} on {ns} stream(s), {'graphs' if graphs else 'eager'}: {dt * 1e3:.2f} ms/step, "
+              f"{points / dt:.4g} point-steps/s, {out[key]}", file=sys.stderr, flush=True)
         del ens, states
     return out
 
@@ -361,7 +367,7 @@ def main():
         for k, v in long_runs.items():
             print(f"[bench] long run {k}: {v}", file=sys.stderr, flush=True)
 
-    ens = ensemble_bench(sw, args.ensemble_steps, 1) if args.ensemble_steps > 0 else None
+    ens = ensemble_bench(sw, args.ensemble_steps, 2) if args.ensemble_steps > 0 else None
 
     # headline: the fastest precision mode that meets the accuracy bar against the FP64 step
     meets = all(v <= ACCURACY_BAR for v in err.values())

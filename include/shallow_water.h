@@ -147,6 +147,15 @@ int sw_ensemble_destroy(sw_ensemble* e);
  * iters[3 i ..] and status[i] (0 or a negative sw_status) per member i; returns the first member's
  * non-zero status, if any. */
 int sw_ensemble_advance(sw_ensemble* e, double* diag, int* iters, int* status);
+/* Graph replay: after a member's first step with ggen set, its next step is
+ * recorded as a hipGraph (a snapshot copy, then advance with each psolve given its last iteration count
+ * plus margin gated iterations) and replayed on later steps; a replay whose psolve needed more iterations
+ * is undone from the snapshot and redone eagerly, and the step is recorded again.  Results are
+ * bit-identical either way.  Setting drops the recorded graphs.  Default: off (on MI355X with ROCm 7.2
+ * the replays measure no faster than eager launches), margin 4. */
+int sw_ensemble_set_graphs(sw_ensemble* e, int enable, int margin);
+/* graph replays, eager redos and captures since the ensemble was created */
+int sw_ensemble_stats(sw_ensemble* e, long* replays, long* redos, long* captures);
 
 /* ---- instrumentation and controls (no reference counterpart) ---- */
 /* Time one kind of launch of this handle with HIP events on the library stream:

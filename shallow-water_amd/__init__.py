@@ -351,6 +351,16 @@ class Ensemble:
             _lib.sw_ensemble_destroy(self._h)
             self._h = None
 
+    def set_graphs(self, enable=True, margin=4):
+        """Graph replay of the members' steps (on by default): see sw_ensemble_set_graphs."""
+        _check(_lib.sw_ensemble_set_graphs(self._h, C.c_int(1 if enable else 0), C.c_int(margin)))
+
+    def stats(self):
+        """(graph replays, eager redos, captures) so far."""
+        a, b, c = C.c_long(), C.c_long(), C.c_long()
+        _check(_lib.sw_ensemble_stats(self._h, C.byref(a), C.byref(b), C.byref(c)))
+        return a.value, b.value, c.value
+
     def advance(self):
         """One timestep of every member; returns (monitor values (n, 5), psolve iterations (n, 3),
         status (n,)); raises SwError when a member failed (its status is the error code)."""

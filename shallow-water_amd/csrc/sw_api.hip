@@ -13,6 +13,7 @@
 #include <mutex>
 #include <string>
 #include <thread>
+#include <atomic>
 #include <vector>
 
 #include "shallow_water.h"
@@ -153,13 +154,39 @@ struct sw_state {
     }
 };
 
+// A member's captured step: one advance (with ggen set) recorded as a hipGraph on its stream, psolve's
+// three iterations each given a fixed number of gated iterations (cap); the 16 fields are first copied
+// to snap, so that a step whose psolve needed more iterations than captured can be undone and redone
+// eagerly.
+struct MemberGraph {
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    bool valid = false;
+    int cap[3] = {0, 0, 0};
+    DevBuf<double> snap;
+    void reset() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        exec = nullptr;
+        graph = nullptr;
+        valid = false;
+    }
+    ~MemberGraph() { reset(); }
+};
+
 // An ensemble: member states (each with its own spectral handle, hence its own engine and work buffers)
-// advanced together, spread over a few HIP streams each driven by one host thread.
+// advanced together, spread over a few HIP streams each driven by one host thread; with graphs on, each
+// member's step after the first is a replay of its captured step.
 struct sw_ensemble {
     std::vector<sw_state*> m;
     std::vector<hipStream_t> streams;
     std::vector<std::vector<int>> groups;  // member indices per stream
+    std::vector<std::unique_ptr<MemberGraph>> graphs;
+    int use_graphs = 0;
+    int margin = 4;  // iterations captured beyond the last count, per psolve call
+    long replays = 0, redos = 0, captures = 0;
     ~sw_ensemble() {
+        graphs.clear();
         for (auto s : streams) (void)hipStreamDestroy(s);
     }
 };
@@ -872,6 +899,7 @@ int sw_ensemble_create(int nmember, sw_state* const* members, int nstreams, sw_e
             e->groups[g].push_back(i);
             load[g] += work[i];
         }
+        for (int i = 0; i < nmember; i++) e->graphs.push_back(std::make_unique<MemberGraph>());
         *out = e.release();
         return 0;
     });
@@ -887,6 +915,73 @@ int sw_ensemble_destroy(sw_ensemble* e) {
     });
 }
 
+namespace {
+
+// the 16 fields of a member copied between the state and its snapshot (one launch)
+void copy_fields(hipStream_t st, const Engine<double>::Fields& f, double* snap, size_t N, bool to_snap) {
+    PtrTab<double, SW_NFIELDS> a;
+    for (int k = 0; k < SW_NFIELDS; k++) a.p[k] = f.a[k];
+    each(st, (size_t)SW_NFIELDS * N, [=] __device__(size_t i) {
+        const int k = (int)(i / N);
+        const size_t q = i - (size_t)k * N;
+        if (to_snap) snap[i] = a[k][q];
+        else a[k][q] = snap[i];
+    });
+}
+
+// Record member st's next step (advance with ggen set) as a graph on its engine's stream: the snapshot
+// copy, then the step with each psolve iteration capped at the last count plus margin.
+void capture_step(sw_state* st, Engine<double>& eng, MemberGraph& G, int margin) {
+    G.reset();
+    auto f = st->fields();
+    if (G.snap.n < (size_t)SW_NFIELDS * eng.N) G.snap.alloc((size_t)SW_NFIELDS * eng.N);
+    for (int k = 0; k < 3; k++) G.cap[k] = std::min(100, *eng.hint_slot(k) + margin);
+    // relaxed: other host threads keep launching, allocating and synchronising their own streams
+    SW_HIP(hipStreamBeginCapture(eng.st, hipStreamCaptureModeRelaxed));
+    double tcap = st->t, dg[5];
+    int it[3];
+    int rc = 0;
+    try {
+        copy_fields(eng.st, f, G.snap.p, eng.N, true);
+        eng.capturing = true;
+        for (int k = 0; k < 3; k++) eng.cap_iters[k] = G.cap[k];
+        rc = eng.advance(f, tcap, true, dg, it);
+    } catch (...) {
+        eng.capturing = false;
+        hipGraph_t g = nullptr;
+        (void)hipStreamEndCapture(eng.st, &g);
+        if (g) (void)hipGraphDestroy(g);
+        throw;
+    }
+    eng.capturing = false;
+    SW_HIP(hipStreamEndCapture(eng.st, &G.graph));
+    SW_CHECK(rc == 0, rc, "capture of a step failed");
+    SW_HIP(hipGraphInstantiate(&G.exec, G.graph, nullptr, nullptr, 0));
+    G.valid = true;
+}
+
+}  // namespace
+
+int sw_ensemble_set_graphs(sw_ensemble* e, int enable, int margin) {
+    return guard([&] {
+        SW_CHECK(e && margin >= 0, SW_ERR_ARG, "bad argument");
+        e->use_graphs = enable ? 1 : 0;
+        e->margin = margin;
+        for (auto& g : e->graphs) g->reset();
+        return 0;
+    });
+}
+
+int sw_ensemble_stats(sw_ensemble* e, long* replays, long* redos, long* captures) {
+    return guard([&] {
+        SW_CHECK(e && replays && redos && captures, SW_ERR_ARG, "null argument");
+        *replays = e->replays;
+        *redos = e->redos;
+        *captures = e->captures;
+        return 0;
+    });
+}
+
 int sw_ensemble_advance(sw_ensemble* e, double* diag, int* iters, int* status) {
     return guard([&] {
         SW_CHECK(e, SW_ERR_ARG, "null ensemble");
@@ -895,18 +990,89 @@ int sw_ensemble_advance(sw_ensemble* e, double* diag, int* iters, int* status) {
         std::vector<double> dg((size_t)5 * n, 0.0);
         std::vector<int> it((size_t)3 * n, 0);
         std::vector<std::string> msg(n);
+        std::atomic<long> replays{0}, redos{0};
+        // members whose step is recorded after the worker threads have joined: a capture runs while no
+        // other host thread makes HIP calls (concurrent calls have invalidated captures on this runtime)
+        std::vector<int> to_capture;
+        std::mutex cap_mu;
+        auto want_capture = [&](int i) {
+            std::lock_guard<std::mutex> lk(cap_mu);
+            to_capture.push_back(i);
+        };
         auto run_group = [&](int g) {
             (void)hipSetDevice(g_device);
+            hipStream_t S = e->streams[g];
+            auto fail = [&](int i, const std::exception& x) {
+                const Error* er = dynamic_cast<const Error*>(&x);
+                rc[i] = er ? er->code : SW_ERR_ARG;
+                msg[i] = x.what();
+            };
+            // replays first: every member whose captured step applies, launched back to back
+            std::vector<int> replayed, eager;
             for (int i : e->groups[g]) {
+                sw_state* st = e->m[i];
+                MemberGraph& G = *e->graphs[i];
+                if (e->use_graphs && G.valid && st->ggen) {
+                    try {
+                        SW_HIP(hipGraphLaunch(G.exec, S));
+                        replayed.push_back(i);
+                    } catch (const std::exception& x) {
+                        fail(i, x);
+                    }
+                } else {
+                    eager.push_back(i);
+                }
+            }
+            (void)hipStreamSynchronize(S);
+            for (int i : replayed) {
+                sw_state* st = e->m[i];
+                MemberGraph& G = *e->graphs[i];
+                auto& eng = *st->s->e64;
+                hipStream_t old = eng.st;
+                eng.set_stream(S);
+                try {
+                    PsolveCtrl c[3];
+                    bool ok = true;
+                    for (int k = 0; k < 3; k++) {
+                        copy_d2h(&c[k], eng.ctrl_slot(k), sizeof(PsolveCtrl), S);
+                        ok = ok && c[k].done && c[k].status == 0 && c[k].iters < 100;
+                    }
+                    if (ok) {
+                        eng.diagnose_finish(st->t, &dg[(size_t)5 * i]);
+                        for (int k = 0; k < 3; k++) {
+                            it[(size_t)3 * i + k] = c[k].iters;
+                            *eng.hint_slot(k) = c[k].iters;
+                        }
+                        st->t += 1.0 / (double)st->s->ng;
+                        replays++;
+                        // iteration counts creep up: capture again before they reach the cap
+                        bool near = false;
+                        for (int k = 0; k < 3; k++) near = near || c[k].iters >= G.cap[k];
+                        if (near) want_capture(i);
+                    } else {
+                        // a psolve needed more iterations than captured, or failed: undo the step and redo
+                        // it eagerly (which also reports any error as sw_advance would)
+                        copy_fields(S, st->fields(), G.snap.p, eng.N, false);
+                        G.reset();
+                        redos++;
+                        eager.push_back(i);
+                    }
+                } catch (const std::exception& x) {
+                    fail(i, x);
+                }
+                eng.set_stream(old);
+            }
+            for (int i : eager) {
                 sw_state* st = e->m[i];
                 try {
                     rc[i] = with_state(st, [&](auto& eng, auto f) {
                         hipStream_t old = eng.st;
-                        eng.set_stream(e->streams[g]);
+                        eng.set_stream(S);
                         int r = 0;
                         try {
                             r = eng.advance(f, st->t, st->ggen != 0, &dg[(size_t)5 * i], &it[(size_t)3 * i]);
                             SW_HIP(hipStreamSynchronize(eng.st));
+                            if (r == 0 && e->use_graphs && eng.nz_spec()) want_capture(i);
                         } catch (...) {
                             (void)hipStreamSynchronize(eng.st);
                             eng.set_stream(old);
@@ -915,12 +1081,8 @@ int sw_ensemble_advance(sw_ensemble* e, double* diag, int* iters, int* status) {
                         eng.set_stream(old);
                         return r;
                     });
-                } catch (const Error& x) {
-                    rc[i] = x.code;
-                    msg[i] = x.what();
                 } catch (const std::exception& x) {
-                    rc[i] = SW_ERR_ARG;
-                    msg[i] = x.what();
+                    fail(i, x);
                 }
             }
         };
@@ -928,6 +1090,27 @@ int sw_ensemble_advance(sw_ensemble* e, double* diag, int* iters, int* status) {
         for (int g = 1; g < (int)e->groups.size(); g++) th.emplace_back(run_group, g);
         run_group(0);
         for (auto& t : th) t.join();
+        for (int i : to_capture) {
+            if (rc[i]) continue;
+            sw_state* st = e->m[i];
+            auto& eng = *st->s->e64;
+            int g = 0;
+            for (int k = 0; k < (int)e->groups.size(); k++)
+                if (std::find(e->groups[k].begin(), e->groups[k].end(), i) != e->groups[k].end()) g = k;
+            hipStream_t old = eng.st;
+            eng.set_stream(e->streams[g]);
+            try {
+                capture_step(st, eng, *e->graphs[i], e->margin);
+                e->captures++;
+            } catch (const std::exception& x) {
+                const Error* er = dynamic_cast<const Error*>(&x);
+                rc[i] = er ? er->code : SW_ERR_ARG;
+                msg[i] = x.what();
+            }
+            eng.set_stream(old);
+        }
+        e->replays += replays;
+        e->redos += redos;
         if (diag) std::memcpy(diag, dg.data(), dg.size() * sizeof(double));
         if (iters) std::memcpy(iters, it.data(), it.size() * sizeof(int));
         if (status) std::memcpy(status, rc.data(), rc.size() * sizeof(int));

@@ -70,7 +70,8 @@ struct Acc {
 // Block partials of K quantities over i < n; component c is a sum unless bit c of MAXMASK is set,
 // in which case it is a max.  Partials land in part[blockIdx.x * K + c].
 template <int K, unsigned MAXMASK, class F>
-__global__ void __launch_bounds__(256) k_reduce(size_t n, F f, double* part) {
+__global__ void __launch_bounds__(256) k_reduce(size_t n, F f, double* part, const int* gate) {
+    if (gate && *gate) return;  // a finished device-controlled iteration (see k_psolve_finalize)
     __shared__ double sh[256 / 64][K];
     double acc[K], cmp[K];
 #pragma unroll
@@ -471,6 +472,11 @@ struct Engine {
     // changes slowly from step to step
     int iter_hint[4] = {8, 8, 8, 8};
     int psolve_slot = 3;
+    // graph capture of a step (the ensemble): while capturing, each device-controlled psolve enqueues
+    // exactly cap_iters[slot] gated iterations and its control state is read after the graph has run;
+    // diagnose leaves its partial sums in dpart
+    bool capturing = false;
+    int cap_iters[4] = {0, 0, 0, 0};
     // psolve controls
     double toler = 1.0e-9;
     int last_iters = 0;
@@ -638,13 +644,15 @@ struct Engine {
         dim3 grid(nslab * (ng / plan.B)), block(plan.nt);
         const double bytes = 2.0 * sizeof(T) * (double)n2 * nslab;
         const bool time1 = prof == SW_PROF_PSOLVE_DERIV && site == SITE_PSOLVE_DERIV, time2 = prof == SW_PROF_STRAIGHT;
+        FftPlan<T> pl = plan;
+        pl.gate = gate_;
         if (time1) prof_start(bytes);
-        hipLaunchKernelGGL((fft_fwd_pass<T, true, Pro, NoEpi>), grid, block, lds_bytes, st, plan, nslab, pro, NoEpi{}, t);
+        hipLaunchKernelGGL((fft_fwd_pass<T, true, Pro, NoEpi>), grid, block, lds_bytes, st, pl, nslab, pro, NoEpi{}, t);
         if (time1) prof_stop();
         SW_HIP(hipGetLastError());
         LoadPlain<T> lp{t, n2, ng};
         if (time2) prof_start(bytes);
-        hipLaunchKernelGGL((fft_fwd_pass<T, false, LoadPlain<T>, Epi>), grid, block, lds_bytes, st, plan, nslab, lp, epi,
+        hipLaunchKernelGGL((fft_fwd_pass<T, false, LoadPlain<T>, Epi>), grid, block, lds_bytes, st, pl, nslab, lp, epi,
                            (T*)nullptr);
         if (time2) prof_stop();
         SW_HIP(hipGetLastError());
@@ -668,13 +676,15 @@ struct Engine {
         dim3 grid(nslab * (ng / plan.B)), block(plan.nt);
         const double bytes = 2.0 * sizeof(T) * (double)n2 * nslab;
         const bool time1 = prof == SW_PROF_PSOLVE_DERIV && site == SITE_PSOLVE_DERIV, time2 = prof == SW_PROF_STRAIGHT;
+        FftPlan<T> pl = plan;
+        pl.gate = gate_;
         if (time1) prof_start(bytes);
-        hipLaunchKernelGGL((fft_inv_pass<T, true, Pro, NoEpi>), grid, block, lds_bytes, st, plan, nslab, pro, NoEpi{}, t);
+        hipLaunchKernelGGL((fft_inv_pass<T, true, Pro, NoEpi>), grid, block, lds_bytes, st, pl, nslab, pro, NoEpi{}, t);
         if (time1) prof_stop();
         SW_HIP(hipGetLastError());
         LoadPlain<T> lp{t, n2, ng};
         if (time2) prof_start(bytes);
-        hipLaunchKernelGGL((fft_inv_pass<T, false, LoadPlain<T>, Epi>), grid, block, lds_bytes, st, plan, nslab, lp, epi,
+        hipLaunchKernelGGL((fft_inv_pass<T, false, LoadPlain<T>, Epi>), grid, block, lds_bytes, st, pl, nslab, lp, epi,
                            (T*)nullptr);
         if (time2) prof_stop();
         SW_HIP(hipGetLastError());
@@ -808,10 +818,18 @@ struct Engine {
     LoadPlain<T> ld(const T* a, int lev0 = 0) const { return LoadPlain<T>{a + (size_t)lev0 * n2, n2, ng}; }
     StorePlain<T> stp(T* a, int lev0 = 0) const { return StorePlain<T>{a + (size_t)lev0 * n2, n2, ng}; }
 
+    // block partials of a reduction left in out[] (gated by gate_); returns the block count
+    template <int K, unsigned MAXMASK, class F>
+    int reduce_dev(size_t n, F f, double* out) {
+        int blocks = (int)std::min<size_t>(RED_BLOCKS, (n + 255) / 256);
+        hipLaunchKernelGGL((k_reduce<K, MAXMASK, F>), dim3(blocks), dim3(256), 0, st, n, f, out, gate_);
+        SW_HIP(hipGetLastError());
+        return blocks;
+    }
     template <int K, unsigned MAXMASK, class F>
     void reduce(size_t n, F f, double* out_host) {
         int blocks = (int)std::min<size_t>(RED_BLOCKS, (n + 255) / 256);
-        hipLaunchKernelGGL((k_reduce<K, MAXMASK, F>), dim3(blocks), dim3(256), 0, st, n, f, part.p);
+        hipLaunchKernelGGL((k_reduce<K, MAXMASK, F>), dim3(blocks), dim3(256), 0, st, n, f, part.p, (const int*)nullptr);
         SW_HIP(hipGetLastError());
         std::vector<double> h((size_t)blocks * K);
         copy_d2h(h.data(), part.p, h.size() * sizeof(double), st);
@@ -891,6 +909,10 @@ struct Engine {
         st = s;
         if (sub) sub->st = s;
     }
+    // the device control state of psolve call slot k (0: advance's predictor, 1, 2: its correctors), in
+    // the engine that runs the iteration
+    PsolveCtrl* ctrl_slot(int k) { return (sub ? sub->ctrl.p : ctrl.p) + k; }
+    int* hint_slot(int k) { return sub ? &sub->iter_hint[k] : &iter_hint[k]; }
 
     // ---------------------------------------------------------------- nhswps and the time loop
     // Defined in sw_engine_ops.inc and instantiated once per precision (sw_engine_f64.hip,
@@ -906,11 +928,16 @@ struct Engine {
     bool psolve_solve_wave(const T* SPt, T* SP, T* ps, double* sums);
     void psolve_iterate_device(const T* sp0, const T* sigx, const T* sigy, const T* cpt1, const T* cpt2, T* SP, T* ps,
                                const T* pn, int nitmax, int* iters, double* errp, int* status);
+    int psolve_step(const T* sp0, const T* sigx, const T* sigy, const T* cpt1, const T* cpt2, T* SP, T* ps, double* sums);
     int iterate(const T* sp0, const T* sigx, const T* sigy, const T* cpt1, const T* cpt2, T* SP, T* ps, const T* pn,
                 int nitmax, int* iters_out, double* errp_out);
     int psolve(Fields f, int nitmax, int* iters_out, double* errp_out);
     void source(Fields f, T* sqs, T* sds, T* sgs, bool reuse_derivs = false);
     void diagnose(Fields f, double t, double out[5]);
+    void diagnose_launch(Fields f);
+    void diagnose_finish(double t, double out[5]);
+    DevBuf<double> dpart;
+    int diag_blocks = 0;
     // savegrid's output (src/nhswps/mod.rs:438-652): d3 = ql, d, g, r, w, pn as f32 (6 fields), d2 = the
     // vertically integrated d, zeta, q, g, h as f32 (5 slabs), spec3 = the zeta, delta, gamma spectra
     // (3 x (ng + 1)); host arrays

@@ -134,6 +134,7 @@ struct FftPlan {
     int nt;       // threads per block
     int xcd;      // remap block ids so that neighbouring row blocks share an XCD (and its L2)
     const C2<T>* tw;  // tw[k] = exp(-2 pi i k / n), k < n
+    const int* gate;  // non-null and reading non-zero: the launch is a no-op (a finished psolve iteration)
 };
 
 // One Stockham radix-R pass over all packed sequences of the block, in place in LDS.
@@ -221,6 +222,7 @@ struct StorePlain {
 // hand (s, r, f, value) to the epilogue.
 template <typename T, bool TRANS, class Pro, class Epi>
 __global__ void __launch_bounds__(512) fft_fwd_pass(FftPlan<T> pl, int nslab, Pro pro, Epi epi, T* tout) {
+    if (pl.gate && *pl.gate) return;
     extern __shared__ __align__(16) unsigned char lds_raw[];
     C2<T>* buf = reinterpret_cast<C2<T>*>(lds_raw);
     const int n = pl.n, B = pl.B;
@@ -302,6 +304,7 @@ __global__ void __launch_bounds__(512) fft_fwd_pass(FftPlan<T> pl, int nslab, Pr
 // along j.  TRANS: write element (r, x) to out[r + n*x + n*n*s]; otherwise epi(s, r, x, value).
 template <typename T, bool TRANS, class Pro, class Epi>
 __global__ void __launch_bounds__(512) fft_inv_pass(FftPlan<T> pl, int nslab, Pro pro, Epi epi, T* tout) {
+    if (pl.gate && *pl.gate) return;
     extern __shared__ __align__(16) unsigned char lds_raw[];
     C2<T>* buf = reinterpret_cast<C2<T>*>(lds_raw);
     const int n = pl.n, B = pl.B;

@@ -51,6 +51,32 @@ def test_ensemble_equals_members_advanced_alone(sw):
             assert np.array_equal(x.get(f), y.get(f)), (i, f)
 
 
+@pytest.mark.parametrize("margin", [0, 2])
+def test_ensemble_graph_replays_equal_members_alone(sw, margin):
+    # six steps: the first eager (and recorded), the rest graph replays; with margin 0 a psolve that needs
+    # one more iteration than the last step overruns its captured budget, so the undo-and-redo path runs too
+    a, b = states(sw), states(sw)
+    ens = sw.Ensemble(a, nstreams=2)
+    ens.set_graphs(True, margin)
+    for step in range(6):
+        diag, iters, status = ens.advance()
+        assert not status.any()
+        for i, st in enumerate(b):
+            st.set_time(st.t, True)
+            d, it = sw.advance(st)
+            assert np.array_equal(diag[i], d) and list(iters[i]) == it, (step, i)
+        for st in a:
+            st.set_time(st.t, True)
+    replays, redos, captures = ens.stats()
+    print(f"margin {margin}: {replays} replays, {redos} redos, {captures} captures", flush=True)
+    # members with nz = 6 have no device-controlled psolve and always run eagerly
+    assert replays > 0
+    for i, (x, y) in enumerate(zip(a, b)):
+        assert x.t == y.t
+        for f in ALL:
+            assert np.array_equal(x.get(f), y.get(f)), (i, f)
+
+
 def test_ensemble_member_matches_oracle(sw, oracle):
     ng, nz, k0, seed = 96, 6, 8.0, 21
     f = init(sw, ng, nz, k0, seed)
