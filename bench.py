@@ -308,6 +308,9 @@ def main():
                     help="timed steps of the 64-member ensemble workload reported beside the headline (0: skip)")
     ap.add_argument("--long-steps", type=int, default=1000,
                     help="steps of the mixed-vs-FP64 accuracy run reported beside the one-step errors (0: skip)")
+    ap.add_argument("--ensemble-only", action="store_true",
+                    help="only the 64-member ensemble workload (--ensemble-steps steps on --ensemble-streams streams)")
+    ap.add_argument("--ensemble-streams", type=int, default=4)
     ap.add_argument("--only", choices=["fp64", "mixed"], default=None,
                     help="only advance the state --steps steps in this precision after --warmup steps, with no "
                          "report (the run that rocprofv3 profiles)")
@@ -332,6 +335,10 @@ def main():
               "run with --gpus 1", file=sys.stderr)
         sys.exit(2)
     sw = load_sw()
+    if args.ensemble_only:
+        ens = ensemble_bench(sw, args.ensemble_steps, 2, runs=((args.ensemble_streams, False),))
+        print(json.dumps({"ensemble": ens}), flush=True)
+        return
     ng, nz = args.ng, args.nz
     points = ng * ng * (nz + 1)
     init = initial_fields(sw, ng, nz, args.seed)
