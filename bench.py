@@ -200,20 +200,21 @@ def long_run_errors(sw, ng, nz, init, steps):
             "fp64_monitor_t_zrms2half_zrms_zmax_umax": diag}
 
 
-def ensemble_bench(sw, steps, warmup, nmember=64, runs=((4, False), (4, True), (1, False), (1, True))):
+def ensemble_bench(sw, steps, warmup, nmember=64, runs=((4, "batched"), (1, "batched"), (4, "eager"))):
     """The batched-ensemble workload: nmember members, member m with ng = (64, 128, 256)[m % 3],
     nz = (4, 8)[(m // 3) % 2], k0 = (4, 8, 16)[(m // 6) % 3], seed 1000 + m, seeded k^-3 PV with rms
     zeta = 0.3 f (at f/2 the reference algorithm's pressure iteration diverges for the k0 = 4, ng = 64
     members), mixed precision (FP32 psolve coefficient fields and iteration, FP64 elsewhere), each
-    member with its own spectral handle, advanced together by
-    sw_ensemble_advance on 1 or 4 streams, each member's step replayed from a recorded hipGraph or launched
-    eagerly.  Returns the aggregate point-steps/s per run."""
+    member with its own spectral handle, advanced together by sw_ensemble_advance on 1 or 4 streams:
+    "batched" = the members of each shape (about 11) advanced as one batch (every launch of the step covers
+    all of them), "eager" = one member at a time per stream, "graphs" = one member at a time, replayed
+    from a recorded hipGraph.  Returns the aggregate point-steps/s per run."""
     shapes = [((64, 128, 256)[m % 3], (4, 8)[(m // 3) % 2], (4.0, 8.0, 16.0)[(m // 6) % 3], 1000 + m)
               for m in range(nmember)]
     points = sum(ng * ng * (nz + 1) for ng, nz, _, _ in shapes)
     out = {"members": nmember, "points_per_step": points, "precision": "mixed", "rms_zeta": "0.3 f",
            "shapes": "ng=(64,128,256)[m%3], nz=(4,8)[(m//3)%2], k0=(4,8,16)[(m//6)%3], seed=1000+m"}
-    for ns, graphs in runs:
+    for ns, mode in runs:
         states = []
         for ng, nz, k0, seed in shapes:
             qs, ds = sw.synth_kminus3(ng, nz, seed=seed, k0=k0, zeta_rms=0.3 * 4 * np.pi)
@@ -225,7 +226,8 @@ def ensemble_bench(sw, steps, warmup, nmember=64, runs=((4, False), (4, True), (
             states.append(sw.State(sw.Spectral(ng, nz, sw.MIXED), ggen=True, qs=qs, ds=ds, gs=gs, r=r, u=u, v=v,
                                    zeta=zeta))
         ens = sw.Ensemble(states, nstreams=ns)
-        ens.set_graphs(graphs)
+        ens.set_batching(32 if mode == "batched" else 0)
+        ens.set_graphs(mode == "graphs")
 
         def step():
             for st in states:
@@ -241,10 +243,13 @@ def ensemble_bench(sw, steps, warmup, nmember=64, runs=((4, False), (4, True), (
             step()
         dt = (time.perf_counter() - t0) / steps
         r1 = ens.stats()
-        key = f"streams_{ns}_" + ("graphs" if graphs else "eager")
-        out[key] = {"ms_per_step": dt * 1e3, "point_steps_per_s": points / dt,
-                    "replays": r1[0] - r0[0], "redos": r1[1] - r0[1], "captures": r1[2] - r0[2]}
-        print(f"[bench] ensemble of {nmember} on {ns} stream(s), {'graphs' if graphs else 'eager'}: {dt * 1e3:.2f} ms/step, "
+        key = f"streams_{ns}_{mode}"
+        out[key] = {"ms_per_step": dt * 1e3, "point_steps_per_s": points / dt}
+        if mode == "graphs":
+            out[key].update(replays=r1[0] - r0[0], redos=r1[1] - r0[1], captures=r1[2] - r0[2])
+        if mode == "batched":
+            out[key]["batches"] = ens.batches()[0]
+        print(f"[bench] ensemble of {nmember} on {ns} stream(s), {mode}: {dt * 1e3:.2f} ms/step, "
               f"{points / dt:.4g} point-steps/s, {out[key]}", file=sys.stderr, flush=True)
         del ens, states
     return out
@@ -311,6 +316,7 @@ def main():
     ap.add_argument("--ensemble-only", action="store_true",
                     help="only the 64-member ensemble workload (--ensemble-steps steps on --ensemble-streams streams)")
     ap.add_argument("--ensemble-streams", type=int, default=4)
+    ap.add_argument("--ensemble-mode", choices=("batched", "eager", "graphs"), default="batched")
     ap.add_argument("--only", choices=["fp64", "mixed"], default=None,
                     help="only advance the state --steps steps in this precision after --warmup steps, with no "
                          "report (the run that rocprofv3 profiles)")
@@ -336,7 +342,7 @@ def main():
         sys.exit(2)
     sw = load_sw()
     if args.ensemble_only:
-        ens = ensemble_bench(sw, args.ensemble_steps, 2, runs=((args.ensemble_streams, False),))
+        ens = ensemble_bench(sw, args.ensemble_steps, 2, runs=((args.ensemble_streams, args.ensemble_mode),))
         print(json.dumps({"ensemble": ens}), flush=True)
         return
     ng, nz = args.ng, args.nz

@@ -156,6 +156,15 @@ int sw_ensemble_advance(sw_ensemble* e, double* diag, int* iters, int* status);
 int sw_ensemble_set_graphs(sw_ensemble* e, int enable, int margin);
 /* graph replays, eager redos and captures since the ensemble was created */
 int sw_ensemble_stats(sw_ensemble* e, long* replays, long* redos, long* captures);
+/* Batching: members of the same ng, nz and precision, with nz in {4, 8, 16, 24, 32}, are advanced as
+ * batches of at most max_members (every stage of the step runs for all of them in the same launches, on
+ * member-contiguous copies of their fields; results bit-identical to advancing them alone).  A batch whose
+ * members differ in ggen or psolve tolerance at a step runs them one by one.  A member whose psolve fails
+ * in a batched step has its status set and its fields unspecified, as after a failed sw_advance; the others
+ * complete the step.  0 or 1: off.  Default: 32.  Graph replay (above) applies to unbatched members. */
+int sw_ensemble_set_batching(sw_ensemble* e, int max_members);
+/* the current batches: their count, the members they hold, and the batched steps run so far */
+int sw_ensemble_batches(sw_ensemble* e, int* nbatch, int* members, long* batched_steps);
 
 /* ---- instrumentation and controls (no reference counterpart) ---- */
 /* Time one kind of launch of this handle with HIP events on the library stream:

@@ -355,6 +355,17 @@ class Ensemble:
         """Graph replay of the members' steps (on by default): see sw_ensemble_set_graphs."""
         _check(_lib.sw_ensemble_set_graphs(self._h, C.c_int(1 if enable else 0), C.c_int(margin)))
 
+    def set_batching(self, max_members=32):
+        """Same-shape members advanced as batches of at most max_members (0 or 1: off); see
+        sw_ensemble_set_batching."""
+        _check(_lib.sw_ensemble_set_batching(self._h, C.c_int(max_members)))
+
+    def batches(self):
+        """(number of batches, members in batches, batched steps so far)."""
+        a, b, c = C.c_int(), C.c_int(), C.c_long()
+        _check(_lib.sw_ensemble_batches(self._h, C.byref(a), C.byref(b), C.byref(c)))
+        return a.value, b.value, c.value
+
     def stats(self):
         """(graph replays, eager redos, captures) so far."""
         a, b, c = C.c_long(), C.c_long(), C.c_long()

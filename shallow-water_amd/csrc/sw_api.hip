@@ -9,10 +9,12 @@
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
+#include <map>
 #include <memory>
 #include <mutex>
 #include <string>
 #include <thread>
+#include <tuple>
 #include <atomic>
 #include <vector>
 
@@ -174,18 +176,42 @@ struct MemberGraph {
     ~MemberGraph() { reset(); }
 };
 
+// Members of one ng, nz and precision advanced as one batch (an engine with M members, which runs every
+// stage of the step for all of them in the same launches): their 16 fields are gathered into
+// member-contiguous batch fields before the step and scattered back after it.
+struct MemberBatch {
+    std::vector<int> idx;  // member indices
+    int precision = SW_FP64;
+    std::unique_ptr<Engine<double>> e64;
+    std::unique_ptr<Engine<float>> e32;
+    DevBuf<double> f[SW_NFIELDS];  // M N values each
+    DevBuf<double*> tab;           // tab[m SW_NFIELDS + k]: member m's field k
+    Engine<double>::Fields fields() {
+        Engine<double>::Fields a;
+        for (int k = 0; k < SW_NFIELDS; k++) a.a[k] = f[k].p;
+        return a;
+    }
+};
+
 // An ensemble: member states (each with its own spectral handle, hence its own engine and work buffers)
-// advanced together, spread over a few HIP streams each driven by one host thread; with graphs on, each
-// member's step after the first is a replay of its captured step.
+// advanced together, spread over a few HIP streams each driven by one host thread.  Members of the same
+// shape and precision (nz with a compile-time specialisation) run as batches of at most batch_max; the
+// others one by one, and with graphs on each such member's step after the first is a replay of its
+// captured step.
 struct sw_ensemble {
     std::vector<sw_state*> m;
     std::vector<hipStream_t> streams;
-    std::vector<std::vector<int>> groups;  // member indices per stream
+    std::vector<std::vector<int>> groups;   // unbatched member indices per stream
+    std::vector<std::vector<int>> bgroups;  // batch indices per stream
+    std::vector<std::unique_ptr<MemberBatch>> batches;
     std::vector<std::unique_ptr<MemberGraph>> graphs;
+    int batch_max = 32;
     int use_graphs = 0;
     int margin = 4;  // iterations captured beyond the last count, per psolve call
-    long replays = 0, redos = 0, captures = 0;
+    long replays = 0, redos = 0, captures = 0, batched_steps = 0;
     ~sw_ensemble() {
+        for (auto s : streams) (void)hipStreamSynchronize(s);
+        batches.clear();
         graphs.clear();
         for (auto s : streams) (void)hipStreamDestroy(s);
     }
@@ -864,6 +890,107 @@ int sw_nhswps(const char* dir, size_t ng, size_t nz, double time_step, double du
     });
 }
 
+namespace {
+
+bool batchable_nz(size_t nz) { return nz == 4 || nz == 8 || nz == 16 || nz == 24 || nz == 32; }
+
+// Units of work (batches of same-shape members, single members) to streams: largest first, each to the
+// stream with the least work so far (work ~ points, FP64 counted twice).  Batch engines are created on
+// their stream.
+void partition(sw_ensemble* e) {
+    for (auto s : e->streams) SW_HIP(hipStreamSynchronize(s));
+    e->batches.clear();
+    const int n = (int)e->m.size(), ns = (int)e->streams.size();
+    std::map<std::tuple<size_t, size_t, int>, std::vector<int>> by;
+    std::vector<int> singles;
+    for (int i = 0; i < n; i++) {
+        const sw_spectral* sp = e->m[i]->s;
+        if (e->batch_max >= 2 && batchable_nz(sp->nz)) by[{sp->ng, sp->nz, sp->precision}].push_back(i);
+        else singles.push_back(i);
+    }
+    std::vector<std::vector<int>> bl;  // members of each batch
+    for (auto& kv : by) {
+        const auto& v = kv.second;
+        if (v.size() < 2) {
+            singles.push_back(v[0]);
+            continue;
+        }
+        const size_t nb = (v.size() + e->batch_max - 1) / e->batch_max, base = v.size() / nb, extra = v.size() % nb;
+        size_t at = 0;
+        for (size_t b = 0; b < nb; b++) {
+            const size_t cnt = base + (b < extra ? 1 : 0);
+            bl.emplace_back(v.begin() + at, v.begin() + at + cnt);
+            at += cnt;
+        }
+    }
+    auto work1 = [&](int i) {
+        const sw_spectral* sp = e->m[i]->s;
+        return (double)(sp->ng * sp->ng * (sp->nz + 1)) * (sp->precision == SW_FP64 ? 2.0 : 1.0);
+    };
+    struct Unit {
+        double work;
+        int batch, member;
+    };
+    std::vector<Unit> units;
+    for (int b = 0; b < (int)bl.size(); b++) {
+        double w = 0.0;
+        for (int i : bl[b]) w += work1(i);
+        units.push_back({w, b, -1});
+    }
+    for (int i : singles) units.push_back({work1(i), -1, i});
+    std::stable_sort(units.begin(), units.end(), [](const Unit& a, const Unit& b) { return a.work > b.work; });
+    e->groups.assign(ns, {});
+    e->bgroups.assign(ns, {});
+    std::vector<double> load(ns, 0.0);
+    std::vector<int> bstream(bl.size(), 0);
+    for (const Unit& u : units) {
+        const int g = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+        if (u.batch >= 0) e->bgroups[g].push_back(u.batch), bstream[u.batch] = g;
+        else e->groups[g].push_back(u.member);
+        load[g] += u.work;
+    }
+    for (int b = 0; b < (int)bl.size(); b++) {
+        auto B = std::make_unique<MemberBatch>();
+        B->idx = bl[b];
+        const int M = (int)B->idx.size();
+        const sw_spectral* sp = e->m[B->idx[0]]->s;
+        hipStream_t S = e->streams[bstream[b]];
+        B->precision = sp->precision;
+        B->e64 = std::make_unique<Engine<double>>((int)sp->ng, (int)sp->nz, S, M);
+        if (sp->precision == SW_MIXED) {
+            B->e32 = std::make_unique<Engine<float>>((int)sp->ng, (int)sp->nz, S, M);
+            B->e64->sub = B->e32.get();
+        }
+        for (int k = 0; k < SW_NFIELDS; k++) B->f[k].alloc(B->e64->NM);
+        std::vector<double*> h((size_t)M * SW_NFIELDS);
+        for (int m = 0; m < M; m++)
+            for (int k = 0; k < SW_NFIELDS; k++) h[(size_t)m * SW_NFIELDS + k] = e->m[B->idx[m]]->f64[k].p;
+        B->tab.alloc(h.size());
+        copy_h2d(B->tab.p, h.data(), h.size() * sizeof(double*), S);
+        e->batches.push_back(std::move(B));
+    }
+}
+
+// the members' 16 fields into the batch fields (in) or back (out), one launch
+void batch_copy(hipStream_t st, MemberBatch& B, bool in) {
+    PtrTab<double, SW_NFIELDS> bf;
+    for (int k = 0; k < SW_NFIELDS; k++) bf.p[k] = B.f[k].p;
+    double* const* tab = B.tab.p;
+    const size_t N = B.e64->N, MN = B.e64->NM;
+    each(st, (size_t)SW_NFIELDS * MN, [=] __device__(size_t i) {
+        const int k = (int)(i / MN);
+        const size_t r = i - (size_t)k * MN, m = r / N, q = r - m * N;
+        double* a = tab[m * SW_NFIELDS + k];
+        if (in) bf[k][r] = a[q];
+        else a[q] = bf[k][r];
+    });
+}
+
+// psolve's tolerance of a member (the engine that runs its iteration)
+double member_toler(const sw_state* st) { return st->s->e32 ? st->s->e32->toler : st->s->e64->toler; }
+
+}  // namespace
+
 int sw_ensemble_create(int nmember, sw_state* const* members, int nstreams, sw_ensemble** out) {
     return guard([&] {
         SW_CHECK(out && members && nmember > 0, SW_ERR_ARG, "bad ensemble arguments");
@@ -883,24 +1010,19 @@ int sw_ensemble_create(int nmember, sw_state* const* members, int nstreams, sw_e
             SW_HIP(hipStreamCreateWithFlags(&h, hipStreamNonBlocking));
             e->streams.push_back(h);
         }
-        // largest members first, each to the stream with the least work so far (work ~ points, FP64
-        // counted twice)
-        std::vector<int> order(nmember);
-        std::vector<double> work(nmember), load(ns, 0.0);
-        for (int i = 0; i < nmember; i++) {
-            order[i] = i;
-            const sw_spectral* sp = e->m[i]->s;
-            work[i] = (double)(sp->ng * sp->ng * (sp->nz + 1)) * (sp->precision == SW_FP64 ? 2.0 : 1.0);
-        }
-        std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return work[a] > work[b]; });
-        e->groups.assign(ns, {});
-        for (int i : order) {
-            int g = (int)(std::min_element(load.begin(), load.end()) - load.begin());
-            e->groups[g].push_back(i);
-            load[g] += work[i];
-        }
         for (int i = 0; i < nmember; i++) e->graphs.push_back(std::make_unique<MemberGraph>());
+        partition(e.get());
         *out = e.release();
+        return 0;
+    });
+}
+
+int sw_ensemble_set_batching(sw_ensemble* e, int max_members) {
+    return guard([&] {
+        SW_CHECK(e && max_members >= 0, SW_ERR_ARG, "bad argument");
+        e->batch_max = max_members;
+        for (auto& g : e->graphs) g->reset();
+        partition(e);
         return 0;
     });
 }
@@ -982,6 +1104,18 @@ int sw_ensemble_stats(sw_ensemble* e, long* replays, long* redos, long* captures
     });
 }
 
+int sw_ensemble_batches(sw_ensemble* e, int* nbatch, int* members, long* batched_steps) {
+    return guard([&] {
+        SW_CHECK(e && nbatch && members && batched_steps, SW_ERR_ARG, "null argument");
+        *nbatch = (int)e->batches.size();
+        int c = 0;
+        for (auto& b : e->batches) c += (int)b->idx.size();
+        *members = c;
+        *batched_steps = e->batched_steps;
+        return 0;
+    });
+}
+
 int sw_ensemble_advance(sw_ensemble* e, double* diag, int* iters, int* status) {
     return guard([&] {
         SW_CHECK(e, SW_ERR_ARG, "null ensemble");
@@ -990,7 +1124,7 @@ int sw_ensemble_advance(sw_ensemble* e, double* diag, int* iters, int* status) {
         std::vector<double> dg((size_t)5 * n, 0.0);
         std::vector<int> it((size_t)3 * n, 0);
         std::vector<std::string> msg(n);
-        std::atomic<long> replays{0}, redos{0};
+        std::atomic<long> replays{0}, redos{0}, batched{0};
         // members whose step is recorded after the worker threads have joined: a capture runs while no
         // other host thread makes HIP calls (concurrent calls have invalidated captures on this runtime)
         std::vector<int> to_capture;
@@ -1007,8 +1141,46 @@ int sw_ensemble_advance(sw_ensemble* e, double* diag, int* iters, int* status) {
                 rc[i] = er ? er->code : SW_ERR_ARG;
                 msg[i] = x.what();
             };
-            // replays first: every member whose captured step applies, launched back to back
             std::vector<int> replayed, eager;
+            // batches: one batched step when the members agree on ggen and psolve's tolerance, else their
+            // members one by one
+            for (int b : e->bgroups[g]) {
+                MemberBatch& B = *e->batches[b];
+                const int M = (int)B.idx.size();
+                const sw_state* s0 = e->m[B.idx[0]];
+                bool same = true;
+                for (int i : B.idx) same = same && e->m[i]->ggen == s0->ggen && member_toler(e->m[i]) == member_toler(s0);
+                if (!same) {
+                    for (int i : B.idx) eager.push_back(i);
+                    continue;
+                }
+                try {
+                    auto& eng = *B.e64;
+                    eng.set_stream(S);
+                    eng.toler = s0->s->e64->toler;
+                    if (B.e32) B.e32->toler = member_toler(s0);
+                    std::vector<double> tb(M), db((size_t)5 * M);
+                    std::vector<int> ib((size_t)3 * M), sb(M);
+                    for (int m = 0; m < M; m++) tb[m] = e->m[B.idx[m]]->t;
+                    batch_copy(S, B, true);
+                    eng.advance_batch(B.fields(), tb.data(), s0->ggen != 0, db.data(), ib.data(), sb.data());
+                    batch_copy(S, B, false);
+                    SW_HIP(hipStreamSynchronize(S));
+                    for (int m = 0; m < M; m++) {
+                        const int i = B.idx[m];
+                        std::memcpy(&dg[(size_t)5 * i], &db[(size_t)5 * m], 5 * sizeof(double));
+                        std::memcpy(&it[(size_t)3 * i], &ib[(size_t)3 * m], 3 * sizeof(int));
+                        rc[i] = sb[m];
+                        if (sb[m]) msg[i] = "psolve failed in a batched step";
+                        else e->m[i]->t = tb[m];
+                    }
+                    batched++;
+                } catch (const std::exception& x) {
+                    (void)hipStreamSynchronize(S);
+                    for (int i : B.idx) fail(i, x);
+                }
+            }
+            // replays first: every member whose captured step applies, launched back to back
             for (int i : e->groups[g]) {
                 sw_state* st = e->m[i];
                 MemberGraph& G = *e->graphs[i];
@@ -1111,6 +1283,7 @@ int sw_ensemble_advance(sw_ensemble* e, double* diag, int* iters, int* status) {
         }
         e->replays += replays;
         e->redos += redos;
+        e->batched_steps += batched;
         if (diag) std::memcpy(diag, dg.data(), dg.size() * sizeof(double));
         if (iters) std::memcpy(iters, it.data(), it.size() * sizeof(int));
         if (status) std::memcpy(status, rc.data(), rc.size() * sizeof(int));

@@ -39,8 +39,9 @@ struct PtrTab {
     __device__ __forceinline__ T* operator[](int i) const { return p[i]; }
 };
 
-// Slabs of fftw_deriv_lines in K groups of `per` levels: output slab s = g per + l is the derivative of
-// level l of src[g] along x (bit g of xmask set) or along y, written to level l of dst[g].
+// Slabs of fftw_deriv_lines in K groups of `per` levels for each of M members: output slab
+// s = (g M + m) per + l is the derivative of level l of member m of src[g] along x (bit g of xmask set)
+// or along y, written to level l of member m of dst[g]; member m's levels start at m mstride.
 template <typename T, int K>
 struct DerivIo {
     PtrTab<const T, K> src;
@@ -48,14 +49,23 @@ struct DerivIo {
     int per;
     unsigned xmask;
     size_t nn;
-    __device__ __forceinline__ bool along_x(int s) const { return (xmask >> (s / per)) & 1u; }
+    int M = 1;
+    size_t mstride = 0;
+    __device__ __forceinline__ bool along_x(int s) const { return (xmask >> (s / (per * M))) & 1u; }
+    __device__ __forceinline__ size_t off(int s, int& g) const {
+        g = s / (per * M);
+        const int rr = s - g * per * M, m = rr / per;
+        return (size_t)m * mstride + (size_t)(rr - m * per) * nn;
+    }
     __device__ __forceinline__ const T* in(int s) const {
-        const int g = s / per;
-        return src[g] + (size_t)(s - g * per) * nn;
+        int g;
+        const size_t o = off(s, g);
+        return src[g] + o;
     }
     __device__ __forceinline__ T* out(int s) const {
-        const int g = s / per;
-        return dst[g] + (size_t)(s - g * per) * nn;
+        int g;
+        const size_t o = off(s, g);
+        return dst[g] + o;
     }
 };
 
@@ -68,16 +78,19 @@ struct Acc {
 };
 
 // Block partials of K quantities over i < n; component c is a sum unless bit c of MAXMASK is set,
-// in which case it is a max.  Partials land in part[blockIdx.x * K + c].
+// in which case it is a max (Kahan-compensated sums per thread, then a fixed shuffle tree).
+// Batched: grid (nb, members); block (b, m) reduces f(m, i) over i < n into part[(m nb + b) K + c]
+// and does nothing once member m's gate flag is set.
 template <int K, unsigned MAXMASK, class F>
-__global__ void __launch_bounds__(256) k_reduce(size_t n, F f, double* part, const int* gate) {
-    if (gate && *gate) return;  // a finished device-controlled iteration (see k_psolve_finalize)
+__global__ void __launch_bounds__(256) k_reduce(size_t n, F f, double* part, Gate gate) {
+    const int m = blockIdx.y;
+    if (gate.member_off(m)) return;  // a finished device-controlled iteration (see k_psolve_finalize)
     __shared__ double sh[256 / 64][K];
     double acc[K], cmp[K];
 #pragma unroll
     for (int c = 0; c < K; c++) acc[c] = (MAXMASK >> c) & 1 ? -INFINITY : 0.0, cmp[c] = 0.0;
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
-        Acc<K> a = f(i);
+        Acc<K> a = f(m, i);
 #pragma unroll
         for (int c = 0; c < K; c++) {
             if ((MAXMASK >> c) & 1) {
@@ -105,7 +118,7 @@ __global__ void __launch_bounds__(256) k_reduce(size_t n, F f, double* part, con
         for (int c = 0; c < K; c++) {
             double x = sh[0][c];
             for (int w = 1; w < (int)(blockDim.x >> 6); w++) x = ((MAXMASK >> c) & 1) ? fmax(x, sh[w][c]) : x + sh[w][c];
-            part[blockIdx.x * K + c] = x;
+            part[((size_t)m * gridDim.x + blockIdx.x) * K + c] = x;
         }
     }
 }
@@ -152,14 +165,16 @@ __global__ void __launch_bounds__(256) k_level_sum2(size_t n2, F f, double* part
 
 // Mean flow of main_invert (spectral/mod.rs:409-416): out[c] = -sum_iz w_iz * S_iz,c / ng^2, with
 // the level sums S taken in a fixed order from the partials (deterministic): one thread per (level,
-// component) sums the nb block partials in block order, then the levels are combined in order.
+// component) sums the nb block partials in block order, then the levels are combined in order.  Block
+// m: member m of a batch, whose nlev levels are partial rows m nlev .. m nlev + nlev - 1; out[2 m + c].
 static __global__ void __launch_bounds__(256) k_meanflow_finalize(const double* part, int nb, int nlev, const double* w,
                                                            double dsumi, double* out) {
     __shared__ double ls[2 * (MAX_NZ + 1)];
+    const int m = blockIdx.x;
     for (int t = threadIdx.x; t < 2 * nlev; t += blockDim.x) {
         const int iz = t >> 1, c = t & 1;
         double s = 0.0;
-        for (int b = 0; b < nb; b++) s += part[((size_t)iz * nb + b) * 2 + c];
+        for (int b = 0; b < nb; b++) s += part[(((size_t)m * nlev + iz) * nb + b) * 2 + c];
         ls[t] = s;
     }
     __syncthreads();
@@ -169,25 +184,26 @@ static __global__ void __launch_bounds__(256) k_meanflow_finalize(const double* 
         uio -= w[iz] * ls[2 * iz] * dsumi;
         vio -= w[iz] * ls[2 * iz + 1] * dsumi;
     }
-    out[0] = uio;
-    out[1] = vio;
+    out[2 * m] = uio;
+    out[2 * m + 1] = vio;
 }
 
 // The pointwise fold of psolve's fused source launch (fftw_inv_comb_fwd): for level s and physical
 // point (y, x), S = sp0 + sigx px + cpt1 pt + sigy py + cpt2 ptt from the level's derivative inputs
 // k = 0..3 (x, theta, y, theta-theta; y is the derivative of the theta input along the transform
 // axis), or sp0 + cpt2 ptt at the bottom level (one input) (psolve.rs:101-270).
+// Slab s of a batch is level s mod lv of member s / lv.
 template <typename T>
 struct PsolveComb {
     const T* sp0;
     PtrTab<const T, 4> cf;  // coefficient of input k: sigx, cpt1, sigy, cpt2
-    int n;
+    int n, lv;
     size_t nn;
-    __device__ __forceinline__ int count(int s) const { return s == 0 ? 1 : 4; }
-    __device__ __forceinline__ bool deriv_of_prev(int s, int k) const { return s > 0 && k == 2; }
+    __device__ __forceinline__ int count(int s) const { return s % lv == 0 ? 1 : 4; }
+    __device__ __forceinline__ bool deriv_of_prev(int s, int k) const { return s % lv > 0 && k == 2; }
     __device__ __forceinline__ T init(int s, int x, int y) const { return sp0[(size_t)s * nn + y + (size_t)n * x]; }
     __device__ __forceinline__ T coef(int s, int k, int x, int y) const {
-        return cf[s == 0 ? 3 : k][(size_t)s * nn + y + (size_t)n * x];
+        return cf[s % lv == 0 ? 3 : k][(size_t)s * nn + y + (size_t)n * x];
     }
 };
 
@@ -225,15 +241,18 @@ struct PressureLU {
 // formed on the way down (PressureLU: no per-level factor arrays read from HBM); writes ps and the top
 // level of S (used after the iteration), and returns the Parseval-weighted squared change and squared
 // old value of ps for the residual.  NZC = nz.
+// Member mb of a batch: its ps and sptop at mb * mstride.
 template <typename T, int NZC>
 struct PsolveCol {
     const T* filt;
     double dzisq;
-    T* ps;
-    T* sptop;
+    T* ps0;
+    T* sptop0;
     int n;
-    size_t nn;
-    __device__ __forceinline__ Acc<2> operator()(int kx, int ky, const T* c) const {
+    size_t nn, mstride;
+    __device__ __forceinline__ Acc<2> operator()(int mb, int kx, int ky, const T* c) const {
+        T* ps = ps0 + (size_t)mb * mstride;
+        T* sptop = sptop0 + (size_t)mb * mstride;
         const size_t p = kx + (size_t)n * ky;
         const double wgt = ((kx == 0 || 2 * kx == n) ? 1.0 : 2.0) * ((ky == 0 || 2 * ky == n) ? 1.0 : 2.0);
         const double rx = 2 * kx <= n ? kx : n - kx, ry = 2 * ky <= n ? ky : n - ky;
@@ -284,11 +303,13 @@ struct PsolveCtrl {
     double top2;  // squared norm of the caller's top level of pn (enters the first residual)
 };
 
-// One block of 256 threads: resets the control state, with top2 = the nb block partials of its
-// reduction summed as k_psolve_finalize sums the residual partials (Kahan per thread over a fixed set of
-// blocks, then a fixed shuffle tree).
+// Block m (member m of a batch; 256 threads): resets its control state, with top2 = the nb block
+// partials of its reduction summed as k_psolve_finalize sums the residual partials (Kahan per thread over
+// a fixed set of blocks, then a fixed shuffle tree).
 static __global__ void __launch_bounds__(256) k_psolve_ctrl_init(PsolveCtrl* c, const double* top2_part, int nb) {
     __shared__ double sh[4];
+    c += blockIdx.x;
+    top2_part += (size_t)blockIdx.x * nb;
     double acc = 0.0, cmp = 0.0;
     for (int b = threadIdx.x; b < nb; b += blockDim.x) {
         double y = top2_part[b] - cmp, t = acc + y;
@@ -302,8 +323,9 @@ static __global__ void __launch_bounds__(256) k_psolve_ctrl_init(PsolveCtrl* c, 
     *c = PsolveCtrl{0, 0, 0, 0, 1.0, 1.0, sh[0] + sh[1] + sh[2] + sh[3]};
 }
 
-// One block of 256 threads: the residual sums from the nb block partials of the column solve (Kahan per
-// thread over a fixed set of blocks, then a fixed shuffle tree), then the stopping rule: errp =
+// Block m (member m of a batch; 256 threads): the residual sums from its nb block partials of the column
+// solve (part + 2 m nb; Kahan per thread over a fixed set of blocks, then a fixed shuffle tree), then the
+// stopping rule: errp =
 // sqrt(dsum / (osum + 1e-20)) with the top level of the caller's pn added at the first iteration
 // (c->top2); non-finite or (after the first iteration) errp > 1 stop with an error; errp <= toler or
 // nitmax iterations stop; in the mixed mode also two consecutive iterations from the third on whose
@@ -311,6 +333,8 @@ static __global__ void __launch_bounds__(256) k_psolve_ctrl_init(PsolveCtrl* c, 
 static __global__ void __launch_bounds__(256) k_psolve_finalize(const double* part, int nb, double toler, int nitmax,
                                                          int mixed, PsolveCtrl* c) {
     __shared__ double sh[4][2];
+    c += blockIdx.x;
+    part += (size_t)2 * blockIdx.x * nb;
     if (c->done) return;
     double acc[2] = {0.0, 0.0}, cmp[2] = {0.0, 0.0};
     for (int b = threadIdx.x; b < nb; b += blockDim.x)
@@ -355,6 +379,10 @@ template <typename T>
 struct Engine {
     int ng = 0, nz = 0, L = 0;
     size_t n2 = 0, N = 0;
+    // members of a batch: every field holds M states, member m's (ng, ng, nz+1) block at m N (NM = M N);
+    // each stage runs on all members in the same launches, psolve with per-member control
+    int M = 1;
+    size_t NM = 0;
     bool mixed = false;
     hipStream_t st = nullptr;
     HostOps host;
@@ -463,9 +491,9 @@ struct Engine {
             (void)hipEventDestroy(e.second);
         }
     }
-    // device flag that turns the gated launches into no-ops (set while a device-controlled psolve
-    // loop is being enqueued; null otherwise)
-    const int* gate_ = nullptr;
+    // per-member flags that turn the gated launches into no-ops (set while a device-controlled psolve
+    // loop is being enqueued; flag null otherwise)
+    Gate gate_{};
     DevBuf<PsolveCtrl> ctrl;
     // first batch of a device-controlled psolve loop: the iteration count of the previous call in the
     // same position (advance's predictor and two correctors; slot 3 for any other call), which
@@ -481,11 +509,19 @@ struct Engine {
     double toler = 1.0e-9;
     int last_iters = 0;
     double last_errp = 0.0;
+    // the last psolve call's results per member of the batch (iterations, errp, 0 or an error status)
+    std::vector<int> m_iters, m_status;
+    std::vector<double> m_errp;
 
-    Engine(int ng_, int nz_, hipStream_t s) : ng(ng_), nz(nz_), L(nz_ + 1), st(s) {
+    Engine(int ng_, int nz_, hipStream_t s, int members = 1) : ng(ng_), nz(nz_), L(nz_ + 1), st(s), M(members) {
         mixed = sizeof(T) == 4;
         n2 = (size_t)ng * ng;
         N = n2 * L;
+        NM = N * (size_t)M;
+        SW_CHECK(M >= 1 && (M == 1 || nz_spec()), SW_ERR_ARG, "a batch of members needs nz in {4, 8, 16, 24, 32}");
+        m_iters.assign(M, 0);
+        m_status.assign(M, 0);
+        m_errp.assign(M, 0.0);
         SW_CHECK(ng >= 4 && nz >= 2 && nz <= MAX_NZ, SW_ERR_ARG, "need ng >= 4 and 2 <= nz <= 256");
         SW_CHECK(host.build(ng, nz), SW_ERR_FACTOR, "ng is not a product of 2, 3, 4, 5, 6");
         SW_CHECK(reference_transform_is_dft(host.xf), SW_ERR_UNSUPPORTED,
@@ -524,9 +560,9 @@ struct Engine {
             for (int i = 0; i < nz; i++) h[2 * L + i] = htd1[i], h[2 * L + nz + i] = etd1[i];
             upd(vc, h);
         }
-        part.alloc((size_t)RED_BLOCKS * 4 * (MAX_NZ + 1));
-        tmp.alloc((size_t)9 * L * n2);  // the largest batch (source: 9 levels' worth of slabs)
-        scal.alloc(16);
+        part.alloc((size_t)RED_BLOCKS * 4 * (MAX_NZ + 1) * M);
+        tmp.alloc((size_t)9 * L * n2 * M);  // the largest batch (source: 9 levels' worth of slabs)
+        scal.alloc(std::max(16, 2 * M));
         if (mixed) toler = 1.0e-6;
     }
 
@@ -817,19 +853,25 @@ struct Engine {
     // plain slab accessors (level offset lev0)
     LoadPlain<T> ld(const T* a, int lev0 = 0) const { return LoadPlain<T>{a + (size_t)lev0 * n2, n2, ng}; }
     StorePlain<T> stp(T* a, int lev0 = 0) const { return StorePlain<T>{a + (size_t)lev0 * n2, n2, ng}; }
+    // per consecutive levels lev0 .. lev0 + per - 1 of every member of the batch (slab m per + l)
+    LoadML<T> ldm(const T* a, int lev0, int per) const { return LoadML<T>{a + (size_t)lev0 * n2, per, N, n2, ng}; }
+    StoreML<T> stm(T* a, int lev0, int per) const { return StoreML<T>{a + (size_t)lev0 * n2, per, N, n2, ng}; }
 
-    // block partials of a reduction left in out[] (gated by gate_); returns the block count
+    // per member m of the batch, the block partials of f(m, i) over i < n left in out[] (gated by gate_);
+    // returns the block count per member
     template <int K, unsigned MAXMASK, class F>
     int reduce_dev(size_t n, F f, double* out) {
         int blocks = (int)std::min<size_t>(RED_BLOCKS, (n + 255) / 256);
-        hipLaunchKernelGGL((k_reduce<K, MAXMASK, F>), dim3(blocks), dim3(256), 0, st, n, f, out, gate_);
+        hipLaunchKernelGGL((k_reduce<K, MAXMASK, F>), dim3(blocks, M), dim3(256), 0, st, n, f, out, gate_);
         SW_HIP(hipGetLastError());
         return blocks;
     }
+    // a reduction of f(i) over i < n, finished on the host (one member)
     template <int K, unsigned MAXMASK, class F>
-    void reduce(size_t n, F f, double* out_host) {
+    void reduce(size_t n, F f0, double* out_host) {
         int blocks = (int)std::min<size_t>(RED_BLOCKS, (n + 255) / 256);
-        hipLaunchKernelGGL((k_reduce<K, MAXMASK, F>), dim3(blocks), dim3(256), 0, st, n, f, part.p, (const int*)nullptr);
+        auto f = [=] __device__(int, size_t i) { return f0(i); };
+        hipLaunchKernelGGL((k_reduce<K, MAXMASK, decltype(f)>), dim3(blocks), dim3(256), 0, st, n, f, part.p, Gate{});
         SW_HIP(hipGetLastError());
         std::vector<double> h((size_t)blocks * K);
         copy_d2h(h.data(), part.p, h.size() * sizeof(double), st);
@@ -957,6 +999,11 @@ struct Engine {
     int run(Fields f, double& t, double tsim, double tgsave, double tstep, double* mon, int maxmon, int* nmon,
             double* eco, int maxeco, int* neco, int* iters_total, RunHooks* hooks = nullptr);
     int advance(Fields f, double& t, bool ggen, double diag[5], int iters[3]);
+    // one step of every member of the batch: t[M], diag[5 M], iters[3 M], status[M]; returns the first
+    // member's non-zero status
+    int advance_batch(Fields f, double* t, bool ggen, double* diag, int* iters, int* status);
+    // the monitor values of every member from diagnose's partials: t[M] in, out[5 M]
+    void diagnose_finish_batch(const double* t, double* out);
 };
 
 // The two precisions are instantiated in their own translation units.

@@ -124,6 +124,17 @@ __device__ __forceinline__ void dftR(C2<T>* v) {
 }
 
 // Launch description of one transform length (plain data, passed by value to kernels).
+// The done flags of device-controlled psolve loops, one per member of a batch: a block of a gated launch
+// that works on slab s belongs to member s / spm and does nothing once that member's flag is set (a
+// finished iteration; see k_psolve_finalize).  flag == nullptr: not gated.
+struct Gate {
+    const int* flag = nullptr;  // the first member's flag
+    int stride = 0;             // ints between consecutive members' flags
+    int spm = 1 << 30;          // slabs per member in the launch (default: every slab is member 0's)
+    __device__ __forceinline__ bool member_off(int m) const { return flag && flag[(size_t)m * stride]; }
+    __device__ __forceinline__ bool off(int s) const { return flag && flag[(size_t)(s / spm) * stride]; }
+};
+
 template <typename T>
 struct FftPlan {
     int n;        // transform length
@@ -134,7 +145,7 @@ struct FftPlan {
     int nt;       // threads per block
     int xcd;      // remap block ids so that neighbouring row blocks share an XCD (and its L2)
     const C2<T>* tw;  // tw[k] = exp(-2 pi i k / n), k < n
-    const int* gate;  // non-null and reading non-zero: the launch is a no-op (a finished psolve iteration)
+    Gate gate;        // gated launches: blocks of finished members do nothing
 };
 
 // One Stockham radix-R pass over all packed sequences of the block, in place in LDS.
@@ -217,19 +228,43 @@ struct StorePlain {
     __device__ void operator()(int s, int r, int f, T v) const { a[(size_t)s * slab_stride + (size_t)r * n + f] = v; }
 };
 
+// The same for the slabs of a batch of members: slab s = m per + l is level lev0 + l of member m,
+// whose levels start at m mstride.
+template <typename T>
+struct LoadML {
+    const T* a;
+    int per;
+    size_t mstride, nn;
+    int n;
+    __device__ T operator()(int s, int r, int j) const {
+        const int m = s / per;
+        return a[(size_t)m * mstride + (size_t)(s - m * per) * nn + (size_t)r * n + j];
+    }
+};
+template <typename T>
+struct StoreML {
+    T* a;
+    int per;
+    size_t mstride, nn;
+    int n;
+    __device__ void operator()(int s, int r, int f, T v) const {
+        const int m = s / per;
+        a[(size_t)m * mstride + (size_t)(s - m * per) * nn + (size_t)r * n + f] = v;
+    }
+};
+
 // Forward (real -> Hermitian) pass.  Rows come from the prologue pro(s, r, j); the transform runs
 // along j.  TRANS: write element (r, f) to out[r + n*f + n*n*s] (coalesced through LDS); otherwise
 // hand (s, r, f, value) to the epilogue.
 template <typename T, bool TRANS, class Pro, class Epi>
 __global__ void __launch_bounds__(512) fft_fwd_pass(FftPlan<T> pl, int nslab, Pro pro, Epi epi, T* tout) {
-    if (pl.gate && *pl.gate) return;
     extern __shared__ __align__(16) unsigned char lds_raw[];
     C2<T>* buf = reinterpret_cast<C2<T>*>(lds_raw);
     const int n = pl.n, B = pl.B;
     const int bps = pl.rows / B;
     const int bid = fft_block_id(pl.xcd);
     const int s = bid / bps, r0 = (bid - s * bps) * B;
-    if (s >= nslab) return;
+    if (s >= nslab || pl.gate.off(s)) return;
     const int npair = B > 1 ? B / 2 : 1;
     const int nt = blockDim.x;
 
@@ -304,14 +339,13 @@ __global__ void __launch_bounds__(512) fft_fwd_pass(FftPlan<T> pl, int nslab, Pr
 // along j.  TRANS: write element (r, x) to out[r + n*x + n*n*s]; otherwise epi(s, r, x, value).
 template <typename T, bool TRANS, class Pro, class Epi>
 __global__ void __launch_bounds__(512) fft_inv_pass(FftPlan<T> pl, int nslab, Pro pro, Epi epi, T* tout) {
-    if (pl.gate && *pl.gate) return;
     extern __shared__ __align__(16) unsigned char lds_raw[];
     C2<T>* buf = reinterpret_cast<C2<T>*>(lds_raw);
     const int n = pl.n, B = pl.B;
     const int bps = pl.rows / B;
     const int bid = fft_block_id(pl.xcd);
     const int s = bid / bps, r0 = (bid - s * bps) * B;
-    if (s >= nslab) return;
+    if (s >= nslab || pl.gate.off(s)) return;
     const int npair = B > 1 ? B / 2 : 1;
     const int nt = blockDim.x;
     const int K = n / 2 + 1;

@@ -328,14 +328,13 @@ __global__ void __launch_bounds__(64 * NW) fftw_fwd_dcomb(const C2<T>* __restric
 // to out[r + n x + n n s]; otherwise epi(s, r, x, value).
 template <typename T, int N, int NW, bool TRANS, class Pro, class Epi, bool CUT = false>
 __global__ void __launch_bounds__(64 * NW) fftw_inv_pass(const C2<T>* __restrict__ tw, int nslab, Pro pro, Epi epi,
-                                                        T* tout, const int* gate, int kcut) {
-    if (gate && *gate) return;  // a finished device-controlled iteration (see k_psolve_finalize)
+                                                        T* tout, Gate gate, int kcut) {
     constexpr int E = WaveShape<N>::E, BPS = N / (2 * NW);
     extern __shared__ __align__(16) unsigned char lds_raw[];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     C2<T>* buf = reinterpret_cast<C2<T>*>(lds_raw) + wv * wave_region<N>();
     const int s = blockIdx.x / BPS, r0 = (blockIdx.x - s * BPS) * 2 * NW, ra = r0 + 2 * wv;
-    if (s >= nslab) return;
+    if (s >= nslab || gate.off(s)) return;  // gate: a finished device-controlled iteration
 
     C2<T> v[E];
     wave_load_herm_pair<N, CUT>(v, buf, lane, [&](int h, int f) { return pro(s, ra + h, f); }, kcut);
@@ -380,14 +379,13 @@ __global__ void __launch_bounds__(64 * NW) fftw_inv_pass(const C2<T>* __restrict
 // through HBM.
 template <typename T, int N, int NW, class Pro, class Comb>
 __global__ void __launch_bounds__(64 * NW) fftw_inv_comb_fwd(const C2<T>* __restrict__ tw, int nslab, Pro pro, Comb comb, T* tout,
-                                                            const int* gate) {
-    if (gate && *gate) return;
+                                                            Gate gate) {
     constexpr int E = WaveShape<N>::E, BPS = N / (2 * NW), NT = E / 2 + 1;
     extern __shared__ __align__(16) unsigned char lds_raw[];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     C2<T>* buf = reinterpret_cast<C2<T>*>(lds_raw) + wv * wave_region<N>();
     const int s = blockIdx.x / BPS, r0 = (blockIdx.x - s * BPS) * 2 * NW, ra = r0 + 2 * wv;
-    if (s >= nslab) return;
+    if (s >= nslab || gate.off(s)) return;
 
     C2<T> acc[E];  // rows ra (.x) and ra + 1 (.y) at j = lane + 64 r
 #pragma unroll
@@ -577,21 +575,21 @@ __global__ void __launch_bounds__(64 * NW) fftw_deriv_lines(const C2<T>* __restr
 }
 
 // Forward pass along x of every level of one row, then a per-column functor across the levels.  Block
-// ky owns row ky of the LV slabs ld(l, ky, x) (l < LV); its NWV waves transform the rows as level pairs
-// (pair m on wave m mod NWV), the spectra land in an LDS column store cs[l N + kx], and each thread then
-// calls col(kx, ky, cs + kx) for its columns (level l at offset l N).  The Acc<2> results are summed
-// over the block, in a fixed order, into part[2 ky + c].  A non-null gate that reads non-zero makes
-// the launch a no-op (fftw_inv_pass and fftw_inv_comb_fwd take the same argument).
+// b = mb N + ky (member mb of a batch) owns row ky of the member's LV slabs ld(mb, l, ky, x) (l < LV); its
+// NWV waves transform the rows as level pairs (pair m on wave m mod NWV), the spectra land in an LDS
+// column store cs[l N + kx], and each thread then calls col(mb, kx, ky, cs + kx) for its columns (level
+// l at offset l N).  The Acc<2> results are summed over the block, in a fixed order, into part[2 b + c].
+// The block does nothing once its member's gate flag is set.
 template <typename T, int N, int NWV, int LV, class Ld, class Col>
 __global__ void __launch_bounds__(64 * NWV) fftw_fwd_rows_cols(const C2<T>* __restrict__ tw, Ld ld, Col col,
-                                                              double* part, const int* gate) {
-    if (gate && *gate) return;
+                                                              double* part, Gate gate) {
+    const int mb = blockIdx.x / N, ky = blockIdx.x - mb * N;
+    if (gate.member_off(mb)) return;
     constexpr int E = WaveShape<N>::E, NT = E / 2 + 1, NP = (LV + 1) / 2, MAXP = (NP + NWV - 1) / NWV;
     extern __shared__ __align__(16) unsigned char lds_raw[];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     C2<T>* buf = reinterpret_cast<C2<T>*>(lds_raw) + wv * wave_region<N>();
     T* cs = reinterpret_cast<T*>(lds_raw);
-    const int ky = blockIdx.x;
 
     T ar[MAXP][NT], ai[MAXP][NT], br[MAXP][NT], bi[MAXP][NT];
 #pragma unroll
@@ -603,7 +601,7 @@ __global__ void __launch_bounds__(64 * NWV) fftw_fwd_rows_cols(const C2<T>* __re
 #pragma unroll
             for (int r = 0; r < E; r++) {
                 const int x = lane + 64 * r;
-                v[r] = C2<T>{ld(l0, ky, x), l1 < LV ? ld(l1, ky, x) : T(0)};
+                v[r] = C2<T>{ld(mb, l0, ky, x), l1 < LV ? ld(mb, l1, ky, x) : T(0)};
             }
             wave_stockham<N, false>(v, buf, tw, lane);
             wave_split_pair<N>(v, buf, lane, ar[m], ai[m], br[m], bi[m]);
@@ -632,7 +630,7 @@ __global__ void __launch_bounds__(64 * NWV) fftw_fwd_rows_cols(const C2<T>* __re
     __syncthreads();
     double s0 = 0.0, s1 = 0.0;
     for (int kx = threadIdx.x; kx < N; kx += 64 * NWV) {
-        const auto a = col(kx, ky, cs + kx);
+        const auto a = col(mb, kx, ky, cs + kx);
         s0 += a.v[0];
         s1 += a.v[1];
     }
@@ -647,8 +645,8 @@ __global__ void __launch_bounds__(64 * NWV) fftw_fwd_rows_cols(const C2<T>* __re
     if (threadIdx.x == 0) {
         double a0 = red[0], a1 = red[1];
         for (int w = 1; w < NWV; w++) a0 += red[2 * w], a1 += red[2 * w + 1];
-        part[2 * ky] = a0;
-        part[2 * ky + 1] = a1;
+        part[2 * (size_t)blockIdx.x] = a0;
+        part[2 * (size_t)blockIdx.x + 1] = a1;
     }
 }
 

@@ -95,3 +95,47 @@ def test_ensemble_rejects_shared_spectral_handle(sw):
     with pytest.raises(sw.SwError) as e:
         sw.Ensemble([sw.State(s, ggen=True, **f), sw.State(s, ggen=True, **f)])
     assert e.value.code == -1
+
+
+# same-shape members that the ensemble advances as batches (wave-kernel and block-per-rows transform
+# lengths, FP64 and mixed), plus a member of its own shape
+BATCHED = ([(128, 8, "fp64", k0, 30 + j) for j, k0 in enumerate((4.0, 8.0, 16.0))] +
+           [(128, 4, "mixed", k0, 40 + j) for j, k0 in enumerate((8.0, 16.0, 8.0))] +
+           [(96, 8, "fp64", k0, 50 + j) for j, k0 in enumerate((8.0, 16.0))] +
+           [(64, 4, "mixed", 16.0, 60 + j) for j in range(2)] +
+           [(256, 16, "fp64", 8.0, 70 + j) for j in range(2)] +
+           [(64, 6, "fp64", 8.0, 80)])
+
+
+def batched_states(sw):
+    out = []
+    for ng, nz, prec, k0, seed in BATCHED:
+        p = sw.FP64 if prec == "fp64" else sw.MIXED
+        out.append(sw.State(sw.Spectral(ng, nz, p), ggen=True, **init(sw, ng, nz, k0, seed)))
+    return out
+
+
+@pytest.mark.parametrize("batch_max", [32, 2])
+def test_batched_ensemble_equals_members_alone(sw, batch_max):
+    # three steps; at the third, member 0 runs with ggen off, so its batch runs its members one by one
+    a, b = batched_states(sw), batched_states(sw)
+    ens = sw.Ensemble(a, nstreams=3)
+    ens.set_batching(batch_max)
+    nb, nm, _ = ens.batches()
+    print(f"batch_max {batch_max}: {nb} batches holding {nm} members", flush=True)
+    assert nm == len(BATCHED) - 1 and nb == (5 if batch_max == 32 else 7)
+    for step in range(3):
+        ggen = [not (step == 2 and i == 0) for i in range(len(a))]
+        for st, g in zip(a, ggen):
+            st.set_time(st.t, g)
+        diag, iters, status = ens.advance()
+        assert not status.any()
+        for i, st in enumerate(b):
+            st.set_time(st.t, ggen[i])
+            d, it = sw.advance(st)
+            assert np.array_equal(diag[i], d) and list(iters[i]) == it, (step, i, diag[i], d, iters[i], it)
+    assert ens.batches()[2] == 3 * nb - 1
+    for i, (x, y) in enumerate(zip(a, b)):
+        assert x.t == y.t
+        for f in ALL:
+            assert np.array_equal(x.get(f), y.get(f)), (i, f)
