@@ -166,6 +166,21 @@ int sw_ensemble_set_batching(sw_ensemble* e, int max_members);
 /* the current batches: their count, the members they hold, and the batched steps run so far */
 int sw_ensemble_batches(sw_ensemble* e, int* nbatch, int* members, long* batched_steps);
 
+/* ---- tiles of the block-per-rows transform passes (no reference counterpart; the transform lengths
+ * without wave-per-sequence kernels, i.e. other than 128, 256, 512) ---- */
+/* Rows per block B (B / 2 complex sequences of two packed real rows) and threads per block nt of the
+ * passes of length n over values of `bytes` bytes (8: FP64, 4: FP32), for the engines created after the
+ * call; B = 0 drops the override.  Valid: B divides n, B even (1 for odd n), nt a multiple of 64 up to 512, the
+ * block's B rows within the 160 KB of LDS per workgroup, at most 16 complex values per thread.  Results
+ * do not depend on the tile.  Default: the measured table compiled into the library (autotune.py), else
+ * the largest power-of-two B up to 16 within 64 KB of LDS. */
+int sw_fft_set_tile(size_t n, int bytes, int B, int nt);
+/* the tile an engine created now uses */
+int sw_fft_get_tile(size_t n, int bytes, int* B, int* nt);
+/* Time of one forward and one inverse 2-D transform of nslab plain slabs with tile (B, nt), averaged
+ * over reps calls (microseconds per transform pair); the autotuner's measurement. */
+int sw_fft_time_tile(size_t n, int bytes, int B, int nt, int nslab, int reps, double* us);
+
 /* ---- instrumentation and controls (no reference counterpart) ---- */
 /* Time one kind of launch of this handle with HIP events on the library stream:
  * SW_PROF_STRAIGHT = the straight (second) pass of every 2-D transform, SW_PROF_PSOLVE_DERIV = the

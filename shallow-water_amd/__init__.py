@@ -98,6 +98,27 @@ def revfft(m, n, x, trig=None, factors=None):
 
 
 # ---------------------------------------------------------------- sta2dfft (src/sta2dfft.rs)
+def fft_set_tile(n, bytes_, B, nt):
+    """Tile (rows per block B, threads nt) of the block-per-rows passes of length n over `bytes_`-byte
+    values, for engines created afterwards; B = 0 drops the override (sw_fft_set_tile)."""
+    _check(_lib.sw_fft_set_tile(C.c_size_t(n), C.c_int(bytes_), C.c_int(B), C.c_int(nt)))
+
+
+def fft_get_tile(n, bytes_):
+    """(B, nt) that an engine created now uses for length n."""
+    b, t = C.c_int(), C.c_int()
+    _check(_lib.sw_fft_get_tile(C.c_size_t(n), C.c_int(bytes_), C.byref(b), C.byref(t)))
+    return b.value, t.value
+
+
+def fft_time_tile(n, bytes_, B, nt, nslab, reps):
+    """Microseconds per forward + inverse 2-D transform of nslab slabs with tile (B, nt)."""
+    us = C.c_double()
+    _check(_lib.sw_fft_time_tile(C.c_size_t(n), C.c_int(bytes_), C.c_int(B), C.c_int(nt), C.c_int(nslab),
+                                 C.c_int(reps), C.byref(us)))
+    return us.value
+
+
 class D2FFT:
     def __init__(self, nx, ny, lx=2 * np.pi, ly=2 * np.pi):
         self.nx, self.ny = nx, ny

@@ -313,6 +313,96 @@ static int fft1d(size_t m, size_t n, double* x, const size_t factors[5], bool in
     });
 }
 
+// ---- tiles of the block-per-rows transform passes
+
+int sw_fft_set_tile(size_t n, int bytes, int B, int nt) {
+    return guard([&] {
+        SW_CHECK((bytes == 4 || bytes == 8) && n >= 2 && n <= 65536, SW_ERR_ARG, "bad length or value size");
+        std::lock_guard<std::mutex> lk(tile_mu());
+        if (B == 0) {
+            tile_overrides().erase({(int)n, bytes});
+            return 0;
+        }
+        SW_CHECK(tile_valid((int)n, bytes, B, nt), SW_ERR_ARG, "invalid tile for this length");
+        tile_overrides()[{(int)n, bytes}] = {B, nt};
+        return 0;
+    });
+}
+
+int sw_fft_get_tile(size_t n, int bytes, int* B, int* nt) {
+    return guard([&] {
+        SW_CHECK(B && nt && (bytes == 4 || bytes == 8) && n >= 2 && n <= 65536, SW_ERR_ARG, "bad argument");
+        choose_tile((int)n, bytes, 0, 0, *B, *nt);
+        return 0;
+    });
+}
+
+}  // extern "C"
+
+namespace {
+
+// fwd2d then inv2d of nslab plain slabs with the passes of tile (B, nt), as the engine launches them
+// (transposing first pass, straight second pass); returns microseconds per transform pair
+template <typename T>
+double time_tile(int n, int B, int nt, int nslab, int reps, hipStream_t st) {
+    FftPlan<T> pl{};
+    DevBuf<C2<T>> tw;
+    size_t lds = 0;
+    Engine<T>::make_plan(n, pl, tw, lds, B, nt);
+    const size_t n2 = (size_t)n * n, total = n2 * (size_t)nslab;
+    DevBuf<T> a(total), t(total);
+    T* ap = a.p;
+    each(st, total, [=] __device__(size_t i) { ap[i] = T((double)((i * 2654435761u) % 1000003u) / 1000003.0 - 0.5); });
+    using NoEpi = typename Engine<T>::NoEpi;
+    const void* k[4] = {(const void*)fft_fwd_pass<T, true, LoadPlain<T>, NoEpi>,
+                        (const void*)fft_fwd_pass<T, false, LoadPlain<T>, StorePlain<T>>,
+                        (const void*)fft_inv_pass<T, true, LoadPlain<T>, NoEpi>,
+                        (const void*)fft_inv_pass<T, false, LoadPlain<T>, StorePlain<T>>};
+    for (const void* f : k) lds_optin(f, lds);
+    dim3 grid((unsigned)(nslab * (n / B))), block((unsigned)nt);
+    const LoadPlain<T> la{a.p, n2, n}, lt{t.p, n2, n};
+    const StorePlain<T> sa{a.p, n2, n};
+    auto run = [&]() {
+        hipLaunchKernelGGL((fft_fwd_pass<T, true, LoadPlain<T>, NoEpi>), grid, block, lds, st, pl, nslab, la, NoEpi{}, t.p);
+        hipLaunchKernelGGL((fft_fwd_pass<T, false, LoadPlain<T>, StorePlain<T>>), grid, block, lds, st, pl, nslab, lt, sa,
+                           (T*)nullptr);
+        hipLaunchKernelGGL((fft_inv_pass<T, true, LoadPlain<T>, NoEpi>), grid, block, lds, st, pl, nslab, la, NoEpi{}, t.p);
+        hipLaunchKernelGGL((fft_inv_pass<T, false, LoadPlain<T>, StorePlain<T>>), grid, block, lds, st, pl, nslab, lt, sa,
+                           (T*)nullptr);
+        SW_HIP(hipGetLastError());
+    };
+    run();
+    SW_HIP(hipStreamSynchronize(st));
+    hipEvent_t e0, e1;
+    SW_HIP(hipEventCreate(&e0));
+    SW_HIP(hipEventCreate(&e1));
+    SW_HIP(hipEventRecord(e0, st));
+    for (int r = 0; r < reps; r++) run();
+    SW_HIP(hipEventRecord(e1, st));
+    SW_HIP(hipEventSynchronize(e1));
+    float ms = 0.f;
+    SW_HIP(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return 1000.0 * (double)ms / (double)reps;
+}
+
+}  // namespace
+
+extern "C" {
+
+int sw_fft_time_tile(size_t n, int bytes, int B, int nt, int nslab, int reps, double* us) {
+    return guard([&] {
+        SW_CHECK(us && nslab > 0 && reps > 0 && (bytes == 4 || bytes == 8), SW_ERR_ARG, "bad argument");
+        SW_CHECK(n >= 4 && n <= 65536 && tile_valid((int)n, bytes, B, nt), SW_ERR_ARG, "invalid tile for this length");
+        std::vector<int> r = gpu_radices((int)n);
+        SW_CHECK(!r.empty(), SW_ERR_FACTOR, "unsupported transform length");
+        hipStream_t st = stream();
+        *us = bytes == 8 ? time_tile<double>((int)n, B, nt, nslab, reps, st) : time_tile<float>((int)n, B, nt, nslab, reps, st);
+        return 0;
+    });
+}
+
 int sw_forfft(size_t m, size_t n, double* x, const double* trig, const size_t factors[5]) {
     (void)trig;
     return fft1d(m, n, x, factors, false);

@@ -9,6 +9,7 @@
 #include <algorithm>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <string>
 #include <type_traits>
 #include <climits>
@@ -68,6 +69,75 @@ struct DerivIo {
         return dst[g] + o;
     }
 };
+
+// Tile shapes of the block-per-rows transform passes: B rows per block (B / 2 complex sequences of two
+// packed real rows; B = 1, odd lengths only: one real row), nt threads per block.  Measured per transform length and value
+// size by autotune.py (which times the candidates with sw_fft_time_tile) and compiled in from
+// sw_fft_tiles.inc; sw_fft_set_tile overrides an entry for the engines created afterwards.  Lengths with
+// no entry use the rule of choose_tile (the largest power-of-two B up to 16 within 64 KB of LDS).
+struct FftTile {
+    int n, bytes, B, nt;
+};
+#include "sw_fft_tiles.inc"  // kFftTiles[]
+
+constexpr size_t FFT_LDS_MAX = 163840;  // LDS per workgroup on gfx950
+
+inline bool tile_valid(int n, int bytes, int B, int nt) {
+    // rows are transformed in pairs (2p, 2p + 1) packed as one complex sequence; the pairing sets the
+    // rounding, so even lengths take even B (B = 1, one row alone, only for odd lengths)
+    if (B < 1 || n % B != 0 || (n % 2 == 0 ? B % 2 != 0 : B != 1) || nt < 64 || nt > 512 || nt % 64 != 0) return false;
+    const int npair = B > 1 ? B / 2 : 1;
+    return (size_t)std::max(B, 2) * n * bytes <= FFT_LDS_MAX && (size_t)npair * n <= (size_t)FFT_EPT_MAX * nt;
+}
+
+inline std::mutex& tile_mu() {
+    static std::mutex m;
+    return m;
+}
+inline std::map<std::pair<int, int>, std::pair<int, int>>& tile_overrides() {
+    static std::map<std::pair<int, int>, std::pair<int, int>> m;
+    return m;
+}
+
+// The tile of the passes of length n over values of `bytes` bytes: forced values if given, else the
+// override, else the measured table, else the rule.
+inline void choose_tile(int n, int bytes, int force_B, int force_nt, int& B, int& nt) {
+    B = nt = 0;
+    if (!force_B && !force_nt) {
+        std::lock_guard<std::mutex> lk(tile_mu());
+        auto it = tile_overrides().find({n, bytes});
+        if (it != tile_overrides().end()) {
+            B = it->second.first, nt = it->second.second;
+        } else {
+            for (const FftTile& t : kFftTiles)
+                if (t.n == n && t.bytes == bytes && tile_valid(n, bytes, t.B, t.nt)) B = t.B, nt = t.nt;
+        }
+        if (B) return;
+    }
+    B = 1;
+    for (int b = 16; b >= 2; b /= 2)
+        if (n % b == 0 && (size_t)b * n * bytes <= 65536 && (b / 2) * n <= FFT_EPT_MAX * 512) {
+            B = b;
+            break;
+        }
+    if (force_B) B = force_B;
+    const int npair = B > 1 ? B / 2 : 1;
+    nt = ((npair * n + FFT_EPT_MAX - 1) / FFT_EPT_MAX + 63) / 64 * 64;
+    nt = std::max(64, std::min(512, nt));
+    if (force_nt) nt = force_nt;
+}
+
+// Kernels launched with more than 64 KB of dynamic LDS are allowed it once per kernel.
+inline void lds_optin(const void* kernel, size_t bytes) {
+    if (bytes <= 65536) return;
+    static std::mutex mu;
+    static std::map<const void*, size_t> done;
+    std::lock_guard<std::mutex> lk(mu);
+    size_t& d = done[kernel];
+    if (d >= bytes) return;
+    SW_HIP(hipFuncSetAttribute(kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
+    d = bytes;
+}
 
 constexpr int RED_BLOCKS = 512;
 constexpr int MAX_NZ = 256;
@@ -513,7 +583,7 @@ struct Engine {
     std::vector<int> m_iters, m_status;
     std::vector<double> m_errp;
 
-    Engine(int ng_, int nz_, hipStream_t s, int members = 1) : ng(ng_), nz(nz_), L(nz_ + 1), st(s), M(members) {
+    Engine(int ng_, int nz_, hipStream_t s, int members = 1) : ng(ng_), nz(nz_), L(nz_ + 1), M(members), st(s) {
         mixed = sizeof(T) == 4;
         n2 = (size_t)ng * ng;
         N = n2 * L;
@@ -574,18 +644,10 @@ struct Engine {
         pl.rows = n;
         pl.nrad = (int)r.size();
         for (int i = 0; i < pl.nrad; i++) pl.rad[i] = r[i];
-        const size_t lds_cap = 65536;
-        int B = 1;
-        for (int b = 16; b >= 2; b /= 2)
-            if (n % b == 0 && (size_t)b * n * sizeof(T) <= lds_cap && (b / 2) * n <= FFT_EPT_MAX * 512) {
-                B = b;
-                break;
-            }
-        if (force_B) B = force_B;
-        int npair = B > 1 ? B / 2 : 1;
-        int nt = ((npair * n + FFT_EPT_MAX - 1) / FFT_EPT_MAX + 63) / 64 * 64;
-        nt = std::max(64, std::min(512, nt));
-        if (force_nt) nt = force_nt;
+        int B, nt;
+        choose_tile(n, (int)sizeof(T), force_B, force_nt, B, nt);
+        SW_CHECK(tile_valid(n, (int)sizeof(T), B, nt) || (force_B && !force_nt), SW_ERR_ARG, "invalid FFT tile");
+        const int npair = B > 1 ? B / 2 : 1;
         SW_CHECK(npair * n <= FFT_EPT_MAX * nt, SW_ERR_ARG, "FFT tile too large for the thread count");
         pl.B = B;
         pl.nt = nt;
@@ -682,6 +744,8 @@ struct Engine {
         const bool time1 = prof == SW_PROF_PSOLVE_DERIV && site == SITE_PSOLVE_DERIV, time2 = prof == SW_PROF_STRAIGHT;
         FftPlan<T> pl = plan;
         pl.gate = gate_;
+        lds_optin((const void*)fft_fwd_pass<T, true, Pro, NoEpi>, lds_bytes);
+        lds_optin((const void*)fft_fwd_pass<T, false, LoadPlain<T>, Epi>, lds_bytes);
         if (time1) prof_start(bytes);
         hipLaunchKernelGGL((fft_fwd_pass<T, true, Pro, NoEpi>), grid, block, lds_bytes, st, pl, nslab, pro, NoEpi{}, t);
         if (time1) prof_stop();
@@ -714,6 +778,8 @@ struct Engine {
         const bool time1 = prof == SW_PROF_PSOLVE_DERIV && site == SITE_PSOLVE_DERIV, time2 = prof == SW_PROF_STRAIGHT;
         FftPlan<T> pl = plan;
         pl.gate = gate_;
+        lds_optin((const void*)fft_inv_pass<T, true, Pro, NoEpi>, lds_bytes);
+        lds_optin((const void*)fft_inv_pass<T, false, LoadPlain<T>, Epi>, lds_bytes);
         if (time1) prof_start(bytes);
         hipLaunchKernelGGL((fft_inv_pass<T, true, Pro, NoEpi>), grid, block, lds_bytes, st, pl, nslab, pro, NoEpi{}, t);
         if (time1) prof_stop();

@@ -72,3 +72,19 @@ def test_read_r8_reads_values_after_the_header_and_checks_the_size(sw, tmp_path)
     assert e.value.code == -1 and "expected an 8-byte header" in str(e.value)
     with pytest.raises(sw.SwError):
         sw.read_r8(str(tmp_path / "missing.r8"), 1)
+
+
+def test_fft_tile_override_is_validated(sw):
+    # host-only: the tile table and its overrides (no device work)
+    rule = sw.fft_get_tile(96, 8)
+    assert rule[0] >= 1 and rule[1] % 64 == 0
+    for bad in ((5, 64), (1, 128), (4, 96), (4, 576), (6, 64 * 9)):
+        with pytest.raises(sw.SwError):
+            sw.fft_set_tile(96, 8, *bad)
+    try:
+        sw.fft_set_tile(96, 8, 2, 128)
+        assert sw.fft_get_tile(96, 8) == (2, 128)
+        assert sw.fft_get_tile(96, 4) != (2, 128) or rule == (2, 128)
+    finally:
+        sw.fft_set_tile(96, 8, 0, 0)
+    assert sw.fft_get_tile(96, 8) == rule
