@@ -200,7 +200,7 @@ def long_run_errors(sw, ng, nz, init, steps):
             "fp64_monitor_t_zrms2half_zrms_zmax_umax": diag}
 
 
-def ensemble_bench(sw, steps, warmup, nmember=64, runs=((4, "batched"), (1, "batched"), (4, "eager"))):
+def ensemble_bench(sw, steps, warmup, nmember=64, runs=((0, "batched"), (1, "batched"), (4, "eager"))):
     """The batched-ensemble workload: nmember members, member m with ng = (64, 128, 256)[m % 3],
     nz = (4, 8)[(m // 3) % 2], k0 = (4, 8, 16)[(m // 6) % 3], seed 1000 + m, seeded k^-3 PV with rms
     zeta = 0.3 f (at f/2 the reference algorithm's pressure iteration diverges for the k0 = 4, ng = 64
@@ -243,13 +243,14 @@ def ensemble_bench(sw, steps, warmup, nmember=64, runs=((4, "batched"), (1, "bat
             step()
         dt = (time.perf_counter() - t0) / steps
         r1 = ens.stats()
-        key = f"streams_{ns}_{mode}"
+        nst = ens.batches()[3]
+        key = f"streams_{nst}_{mode}"
         out[key] = {"ms_per_step": dt * 1e3, "point_steps_per_s": points / dt}
         if mode == "graphs":
             out[key].update(replays=r1[0] - r0[0], redos=r1[1] - r0[1], captures=r1[2] - r0[2])
         if mode == "batched":
             out[key]["batches"] = ens.batches()[0]
-        print(f"[bench] ensemble of {nmember} on {ns} stream(s), {mode}: {dt * 1e3:.2f} ms/step, "
+        print(f"[bench] ensemble of {nmember} on {nst} stream(s), {mode}: {dt * 1e3:.2f} ms/step, "
               f"{points / dt:.4g} point-steps/s, {out[key]}", file=sys.stderr, flush=True)
         del ens, states
     return out
@@ -315,7 +316,7 @@ def main():
                     help="steps of the mixed-vs-FP64 accuracy run reported beside the one-step errors (0: skip)")
     ap.add_argument("--ensemble-only", action="store_true",
                     help="only the 64-member ensemble workload (--ensemble-steps steps on --ensemble-streams streams)")
-    ap.add_argument("--ensemble-streams", type=int, default=4)
+    ap.add_argument("--ensemble-streams", type=int, default=0, help="0: one per batch or single member")
     ap.add_argument("--ensemble-mode", choices=("batched", "eager", "graphs"), default="batched")
     ap.add_argument("--only", choices=["fp64", "mixed"], default=None,
                     help="only advance the state --steps steps in this precision after --warmup steps, with no "

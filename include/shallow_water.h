@@ -139,8 +139,8 @@ int sw_state_load(sw_state* st, const char* path);
  * the batched workload) ---- */
 typedef struct sw_ensemble sw_ensemble;
 /* members: nmember states, each with its own spectral handle (any ng, nz >= 3, precision); the
- * ensemble does not own them.  nstreams: HIP streams the members are spread over, each driven by one
- * host thread (0: 4). */
+ * ensemble does not own them.  nstreams: HIP streams the units of work (batches, below, and single
+ * members) are spread over, each driven by one host thread (0: one per unit, at most 8). */
 int sw_ensemble_create(int nmember, sw_state* const* members, int nstreams, sw_ensemble** out);
 int sw_ensemble_destroy(sw_ensemble* e);
 /* One advance step (advance.rs:13) of every member, as sw_advance does for one state: diag[5 i ..],
@@ -163,8 +163,9 @@ int sw_ensemble_stats(sw_ensemble* e, long* replays, long* redos, long* captures
  * in a batched step has its status set and its fields unspecified, as after a failed sw_advance; the others
  * complete the step.  0 or 1: off.  Default: 32.  Graph replay (above) applies to unbatched members. */
 int sw_ensemble_set_batching(sw_ensemble* e, int max_members);
-/* the current batches: their count, the members they hold, and the batched steps run so far */
-int sw_ensemble_batches(sw_ensemble* e, int* nbatch, int* members, long* batched_steps);
+/* the current batches: their count, the members they hold, the batched steps run so far, and the
+ * number of streams */
+int sw_ensemble_batches(sw_ensemble* e, int* nbatch, int* members, long* batched_steps, int* nstreams);
 
 /* ---- tiles of the block-per-rows transform passes (no reference counterpart; the transform lengths
  * without wave-per-sequence kernels, i.e. other than 128, 256, 512) ---- */

@@ -357,10 +357,11 @@ def nhswps(out_dir, ng, nz, time_step, duration, save_interval, precision=FP64):
 
 
 class Ensemble:
-    """Member states advanced together (sw_ensemble_*): each member needs its own Spectral handle; the
-    members are spread over `nstreams` HIP streams, each driven by one host thread."""
+    """Member states advanced together (sw_ensemble_*): each member needs its own Spectral handle; members
+    of one shape and precision run as batches; the batches and single members are spread over `nstreams`
+    HIP streams (0: one per batch or single member, at most 8), each driven by one host thread."""
 
-    def __init__(self, states, nstreams=4):
+    def __init__(self, states, nstreams=0):
         self.states = list(states)
         arr = (C.c_void_p * len(self.states))(*[st._h.value for st in self.states])
         h = _vp()
@@ -382,10 +383,10 @@ class Ensemble:
         _check(_lib.sw_ensemble_set_batching(self._h, C.c_int(max_members)))
 
     def batches(self):
-        """(number of batches, members in batches, batched steps so far)."""
-        a, b, c = C.c_int(), C.c_int(), C.c_long()
-        _check(_lib.sw_ensemble_batches(self._h, C.byref(a), C.byref(b), C.byref(c)))
-        return a.value, b.value, c.value
+        """(number of batches, members in batches, batched steps so far, streams)."""
+        a, b, c, d = C.c_int(), C.c_int(), C.c_long(), C.c_int()
+        _check(_lib.sw_ensemble_batches(self._h, C.byref(a), C.byref(b), C.byref(c), C.byref(d)))
+        return a.value, b.value, c.value, d.value
 
     def stats(self):
         """(graph replays, eager redos, captures) so far."""

@@ -206,6 +206,7 @@ struct sw_ensemble {
     std::vector<std::unique_ptr<MemberBatch>> batches;
     std::vector<std::unique_ptr<MemberGraph>> graphs;
     int batch_max = 32;
+    int nstreams_req = 0;  // 0: one stream per unit of work (batch or single member), at most 8
     int use_graphs = 0;
     int margin = 4;  // iterations captured beyond the last count, per psolve call
     long replays = 0, redos = 0, captures = 0, batched_steps = 0;
@@ -990,7 +991,7 @@ bool batchable_nz(size_t nz) { return nz == 4 || nz == 8 || nz == 16 || nz == 24
 void partition(sw_ensemble* e) {
     for (auto s : e->streams) SW_HIP(hipStreamSynchronize(s));
     e->batches.clear();
-    const int n = (int)e->m.size(), ns = (int)e->streams.size();
+    const int n = (int)e->m.size();
     std::map<std::tuple<size_t, size_t, int>, std::vector<int>> by;
     std::vector<int> singles;
     for (int i = 0; i < n; i++) {
@@ -1012,6 +1013,19 @@ void partition(sw_ensemble* e) {
             bl.emplace_back(v.begin() + at, v.begin() + at + cnt);
             at += cnt;
         }
+    }
+    // streams: as requested, else one per unit of work (a batch's launches and its host synchronisations
+    // then overlap those of the others), at most 8
+    const int units_n = (int)(bl.size() + singles.size());
+    const int ns = std::max(1, std::min(e->nstreams_req > 0 ? e->nstreams_req : std::min(units_n, 8), n));
+    while ((int)e->streams.size() > ns) {
+        (void)hipStreamDestroy(e->streams.back());
+        e->streams.pop_back();
+    }
+    while ((int)e->streams.size() < ns) {
+        hipStream_t h;
+        SW_HIP(hipStreamCreateWithFlags(&h, hipStreamNonBlocking));
+        e->streams.push_back(h);
     }
     auto work1 = [&](int i) {
         const sw_spectral* sp = e->m[i]->s;
@@ -1092,14 +1106,8 @@ int sw_ensemble_create(int nmember, sw_state* const* members, int nstreams, sw_e
                 SW_CHECK(members[j]->s != st->s, SW_ERR_ARG, "ensemble members must not share a spectral handle");
             e->m.push_back(st);
         }
-        int ns = nstreams > 0 ? nstreams : 4;
-        ns = std::max(1, std::min(ns, nmember));
+        e->nstreams_req = std::max(0, nstreams);
         stream();  // the library device is selected
-        for (int g = 0; g < ns; g++) {
-            hipStream_t h;
-            SW_HIP(hipStreamCreateWithFlags(&h, hipStreamNonBlocking));
-            e->streams.push_back(h);
-        }
         for (int i = 0; i < nmember; i++) e->graphs.push_back(std::make_unique<MemberGraph>());
         partition(e.get());
         *out = e.release();
@@ -1194,9 +1202,10 @@ int sw_ensemble_stats(sw_ensemble* e, long* replays, long* redos, long* captures
     });
 }
 
-int sw_ensemble_batches(sw_ensemble* e, int* nbatch, int* members, long* batched_steps) {
+int sw_ensemble_batches(sw_ensemble* e, int* nbatch, int* members, long* batched_steps, int* nstreams) {
     return guard([&] {
-        SW_CHECK(e && nbatch && members && batched_steps, SW_ERR_ARG, "null argument");
+        SW_CHECK(e && nbatch && members && batched_steps && nstreams, SW_ERR_ARG, "null argument");
+        *nstreams = (int)e->streams.size();
         *nbatch = (int)e->batches.size();
         int c = 0;
         for (auto& b : e->batches) c += (int)b->idx.size();

@@ -118,12 +118,14 @@ def batched_states(sw):
 @pytest.mark.parametrize("batch_max", [32, 2])
 def test_batched_ensemble_equals_members_alone(sw, batch_max):
     # three steps; at the third, member 0 runs with ggen off, so its batch runs its members one by one
+    # batch_max 32: one stream per batch or single member (the default); 2: three streams
     a, b = batched_states(sw), batched_states(sw)
-    ens = sw.Ensemble(a, nstreams=3)
+    ens = sw.Ensemble(a, nstreams=0 if batch_max == 32 else 3)
     ens.set_batching(batch_max)
-    nb, nm, _ = ens.batches()
-    print(f"batch_max {batch_max}: {nb} batches holding {nm} members", flush=True)
+    nb, nm, _, nst = ens.batches()
+    print(f"batch_max {batch_max}: {nb} batches holding {nm} members, {nst} streams", flush=True)
     assert nm == len(BATCHED) - 1 and nb == (5 if batch_max == 32 else 7)
+    assert nst == (nb + 1 if batch_max == 32 else 3)
     for step in range(3):
         ggen = [not (step == 2 and i == 0) for i in range(len(a))]
         for st, g in zip(a, ggen):
