@@ -98,9 +98,9 @@ def time_steps(sw, spec, st, steps, warmup, prof_modes):
 
 def measured_traffic(site, kernel, ng, nz, precision):
     """HBM bytes per launch of a profiled kernel from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE
-    readings (profiles/hbm_traffic_<site>_ng<ng>_nz<nz>.txt), scaled by the copy-kernel calibration;
-    None when no reading for this kernel and workload is on file."""
-    path = os.path.join(ROOT, "profiles", f"hbm_traffic_{site}_ng{ng}_nz{nz}.txt")
+    readings (profiles/hbm_traffic_<site>_ng<ng>_nz<nz>_<precision>.txt), scaled by the copy-kernel
+    calibration; None when no reading for this kernel and workload is on file."""
+    path = os.path.join(ROOT, "profiles", f"hbm_traffic_{site}_ng{ng}_nz{nz}_{precision}.txt")
     if not os.path.exists(path):
         return None
     kv = {}
