@@ -268,11 +268,22 @@ __global__ void __launch_bounds__(512) fft_fwd_pass(FftPlan<T> pl, int nslab, Pr
     const int npair = B > 1 ? B / 2 : 1;
     const int nt = blockDim.x;
 
-    for (int e = threadIdx.x; e < npair * n; e += nt) {
-        int p = e / n, j = e - p * n;
-        T a = pro(s, r0 + 2 * p, j);
-        T b = (B > 1) ? pro(s, r0 + 2 * p + 1, j) : T(0);
-        buf[p * n + j] = C2<T>{a, b};
+    // every load of the thread issued before the first LDS store (npair n <= FFT_EPT_MAX nt), so that
+    // their latencies overlap instead of adding up
+    T la[FFT_EPT_MAX], lb[FFT_EPT_MAX];
+#pragma unroll
+    for (int it = 0; it < FFT_EPT_MAX; it++) {
+        const int e = threadIdx.x + it * nt;
+        if (e < npair * n) {
+            const int p = e / n, j = e - p * n;
+            la[it] = pro(s, r0 + 2 * p, j);
+            lb[it] = (B > 1) ? pro(s, r0 + 2 * p + 1, j) : T(0);
+        }
+    }
+#pragma unroll
+    for (int it = 0; it < FFT_EPT_MAX; it++) {
+        const int e = threadIdx.x + it * nt;
+        if (e < npair * n) buf[e] = C2<T>{la[it], lb[it]};
     }
     __syncthreads();
     fft_lds<false>(buf, pl, npair);
@@ -350,19 +361,32 @@ __global__ void __launch_bounds__(512) fft_inv_pass(FftPlan<T> pl, int nslab, Pr
     const int nt = blockDim.x;
     const int K = n / 2 + 1;
 
-    for (int e = threadIdx.x; e < npair * K; e += nt) {
-        int p = e / K, k = e - p * K;
-        int kc = n - k;
-        bool has_s = k > 0 && k < kc;
-        int ra = r0 + 2 * p;
-        T ca = pro(s, ra, k), sa = has_s ? pro(s, ra, kc) : T(0);
-        T cb = T(0), sb = T(0);
-        if (B > 1) {
-            cb = pro(s, ra + 1, k);
-            sb = has_s ? pro(s, ra + 1, kc) : T(0);
+    // every load of the thread issued before the first LDS store (npair K <= (FFT_EPT_MAX / 2 + 1) nt)
+    constexpr int KIT = FFT_EPT_MAX / 2 + 1;
+    T ca[KIT], sa[KIT], cb[KIT], sb[KIT];
+#pragma unroll
+    for (int it = 0; it < KIT; it++) {
+        const int e = threadIdx.x + it * nt;
+        if (e < npair * K) {
+            const int p = e / K, k = e - p * K, kc = n - k, ra = r0 + 2 * p;
+            const bool has_s = k > 0 && k < kc;
+            ca[it] = pro(s, ra, k);
+            sa[it] = has_s ? pro(s, ra, kc) : T(0);
+            cb[it] = T(0), sb[it] = T(0);
+            if (B > 1) {
+                cb[it] = pro(s, ra + 1, k);
+                sb[it] = has_s ? pro(s, ra + 1, kc) : T(0);
+            }
         }
-        buf[p * n + k] = C2<T>{ca - sb, sa + cb};
-        if (has_s) buf[p * n + kc] = C2<T>{ca + sb, cb - sa};
+    }
+#pragma unroll
+    for (int it = 0; it < KIT; it++) {
+        const int e = threadIdx.x + it * nt;
+        if (e < npair * K) {
+            const int p = e / K, k = e - p * K, kc = n - k;
+            buf[p * n + k] = C2<T>{ca[it] - sb[it], sa[it] + cb[it]};
+            if (k > 0 && k < kc) buf[p * n + kc] = C2<T>{ca[it] + sb[it], cb[it] - sa[it]};
+        }
     }
     __syncthreads();
     fft_lds<true>(buf, pl, npair);
