@@ -117,11 +117,14 @@ def measured_traffic(site, kernel, ng, nz, precision):
 
 
 def kernel_names(ng, nz, precision):
+    """The profiled psolve kernels on the fused path (transform lengths 128/256/512, nz 4/8/16/24/32): the
+    derivative rows of the first guess (once per psolve call), and per iteration the source fold (tiled
+    derivative inputs) and the column solve that also forms the next iteration's derivative rows."""
     t = "double" if precision == "fp64" else "float"
-    nwt = 4 if precision == "fp64" else 8  # waves per block of the transposing passes
-    return {"psolve_deriv": f"fftw_inv_pass<{t}, {ng}, {nwt}, true, psolve theta-derivative prologue>",
-            "psolve_comb": f"fftw_inv_comb_fwd<{t}, {ng}, 4, psolve source fold>",
-            "psolve_tri": f"fftw_fwd_rows_cols<{t}, {ng}, 8, {nz + 1}, psolve column solve>"}
+    lv, nd = nz + 1, 1 + 3 * nz
+    return {"psolve_deriv": f"fftw_rows_cols_inv<{t}, {ng}, 8, {lv}, {nd}, 8, load-only: first-guess derivative rows>",
+            "psolve_comb": f"fftw_inv_comb_fwd<{t}, {ng}, 4, psolve source fold, tiled inputs>",
+            "psolve_tri": f"fftw_rows_cols_inv<{t}, {ng}, 8, {lv}, {nd}, 8, psolve column solve + derivative rows>"}
 
 
 def rel_l2(a, b):

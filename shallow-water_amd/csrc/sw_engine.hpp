@@ -277,6 +277,23 @@ struct PsolveComb {
     }
 };
 
+// The tiled derivative inputs of psolve's source fold (fftw_inv_comb_fwd with TILED): input k of level
+// l = s mod lv of member s / lv is slab 0 at the bottom and slab 3 l - 2 + (0, 1, 2) for k = (0, 1, 3)
+// (x derivative, theta, theta-theta; input 2, the y derivative, is formed from input 1), nd slabs per member,
+// in the layout tile_pos<N, RB>; tile() points at row r0, position 0.
+template <typename T, int N, int RB_>
+struct PsolveTiles {
+    static constexpr int RB = RB_;
+    const T* D;
+    int lv, nd;
+    size_t nn;
+    __device__ __forceinline__ const T* tile(int s, int k, int r0) const {
+        const int m = s / lv, l = s - m * lv;
+        const int sl = l == 0 ? 0 : 3 * l - 2 + (k == 0 ? 0 : (k == 1 ? 1 : 2));
+        return D + ((size_t)m * nd + sl) * nn + tile_pos<N, RB>(r0, 0);
+    }
+};
+
 // The LU factors of the pressure tridiagonal at one (kx, ky) (Spectral::new, spectral/mod.rs:214-256):
 // coefficients a0, a0b, ap, apb of rks = kx^2 + ky^2 (integer wavenumbers), then htdv(iz) = filt / (a0 +
 // ap etdv(iz-1)) and etdv(iz) = -ap htdv(iz) level by level.  Contraction is off so that the factors are
@@ -321,6 +338,12 @@ struct PsolveCol {
     int n;
     size_t nn, mstride;
     __device__ __forceinline__ Acc<2> operator()(int mb, int kx, int ky, const T* c) const {
+        T pc[NZC + 1];
+        return (*this)(mb, kx, ky, c, pc);
+    }
+    // the same, leaving the column's new ps (levels 0 .. NZC - 1) and its top level NZC (which the
+    // solve does not change) in pc
+    __device__ __forceinline__ Acc<2> operator()(int mb, int kx, int ky, const T* c, T (&pc)[NZC + 1]) const {
         T* ps = ps0 + (size_t)mb * mstride;
         T* sptop = sptop0 + (size_t)mb * mstride;
         const size_t p = kx + (size_t)n * ky;
@@ -352,12 +375,102 @@ struct PsolveCol {
             dsum += d * d;
             osum += old * old;
             ps[o] = neu;
+            pc[iz] = neu;
         }
+        pc[NZC] = ps[(size_t)NZC * nn + p];
         sptop[p] = c[NZC * n];
         Acc<2> r;
         r.v[0] = wgt * dsum;
         r.v[1] = wgt * osum;
         return r;
+    }
+};
+
+// The theta-derivative spectra of psolve's source read from ps in HBM (psolve.rs:101-124, 127-227,
+// 229-270), one value per call: slab s of member m (sg = m nd + s, nd = 1 + 3 nz) at (kx, ky), with the
+// rows of PsolveDer (the x derivative taken here: dmul(kx) times the value at mir(kx)).
+template <typename T>
+struct PsolveDerivSpec {
+    const T* ps;
+    int n, nzv, nd;
+    size_t nn, NN;
+    T hdzi, dzisq, dz2;
+    __device__ __forceinline__ T operator()(int sg, int ky, int kx) const {
+        const int m = sg / nd, s = sg - m * nd;
+        const T* pm = ps + (size_t)m * NN;
+        size_t p = kx + (size_t)n * ky;
+        auto wka = [&](int iz, size_t q) { return (pm[(size_t)(iz + 1) * nn + q] - pm[(size_t)(iz - 1) * nn + q]) * hdzi; };
+        auto wkd = [&](int iz, size_t q) {
+            size_t o = (size_t)iz * nn + q;
+            return (pm[o + nn] - T(2) * pm[o] + pm[o - nn]) * dzisq;
+        };
+        if (s == 0) return (T(2) * pm[p] - T(5) * pm[nn + p] + T(4) * pm[2 * nn + p] - pm[3 * nn + p]) * dzisq;
+        const int iz = (s + 2) / 3, t = s + 2 - 3 * iz;
+        const size_t qx = mir(kx, n) + (size_t)n * ky;
+        if (iz < nzv) {
+            if (t == 0) return dmul<T>(kx, n) * wka(iz, qx);
+            if (t == 1) return wka(iz, p);
+            return wkd(iz, p);
+        }
+        // top: dpdt(nz) = dpdt(nz-1) + dz2 (3 d2pdt2(nz-1) - d2pdt2(nz-2)), d2pdt2(nz) = 2 d2pdt2(nz-1) - d2pdt2(nz-2)
+        auto w2 = [&](size_t q) { return nzv >= 3 ? wkd(nzv - 2, q) : T(0); };
+        auto top = [&](size_t q) { return wka(nzv - 1, q) + dz2 * (T(3) * wkd(nzv - 1, q) - w2(q)); };
+        if (t == 0) return dmul<T>(kx, n) * top(qx);
+        if (t == 1) return top(p);
+        return T(2) * wkd(nzv - 1, p) - w2(p);
+    }
+};
+
+// The columns of ps (levels 0 .. NZC) of member mb at (kx, ky), loaded from HBM, for the derivative rows of
+// psolve's first guess (fftw_rows_cols_inv without the solve).
+template <typename T, int NZC>
+struct PsolveColLoad {
+    const T* ps0;
+    int n;
+    size_t nn, mstride;
+    __device__ __forceinline__ void load(int mb, int kx, int ky, T (&pc)[NZC + 1]) const {
+        const T* ps = ps0 + (size_t)mb * mstride + kx + (size_t)n * ky;
+#pragma unroll
+        for (int iz = 0; iz <= NZC; iz++) pc[iz] = ps[(size_t)iz * nn];
+    }
+};
+
+// The 1 + 3 NZC theta-derivative rows of psolve's source (psolve.rs:101-124, 127-227, 229-270) from a
+// column of ps (levels 0 .. NZC) held in registers: row 0 = d2p/dtheta2 at the bottom; row 3 iz - 2 + t for
+// iz = 1 .. NZC: t = 0 its x derivative (xd: the derivative along kx is taken when the rows are
+// packed), t = 1 dp/dtheta, t = 2 d2p/dtheta2; at the top (iz = NZC) the extrapolated dp/dtheta =
+// dpdt(nz-1) + dz2 (3 d2pdt2(nz-1) - d2pdt2(nz-2)) and d2p/dtheta2 = 2 d2pdt2(nz-1) - d2pdt2(nz-2), formed
+// spectrally (the extrapolation is linear).
+template <typename T, int NZC>
+struct PsolveDer {
+    T hdzi, dzisq, dz2;
+    static constexpr int ND = 1 + 3 * NZC;
+    __device__ __forceinline__ bool xd(int s) const { return s >= 1 && (s + 2) % 3 == 0; }
+    template <int S>
+    __device__ __forceinline__ T raw(const T (&p)[NZC + 1]) const {
+        static_assert(NZC >= 3 && S >= 0 && S < ND, "row index");
+        auto wka = [&](auto izc) {
+            constexpr int iz = decltype(izc)::value;
+            return (p[iz + 1] - p[iz - 1]) * hdzi;
+        };
+        auto wkd = [&](auto izc) {
+            constexpr int iz = decltype(izc)::value;
+            return (p[iz + 1] - T(2) * p[iz] + p[iz - 1]) * dzisq;
+        };
+        if constexpr (S == 0) {
+            return (T(2) * p[0] - T(5) * p[1] + T(4) * p[2] - p[3]) * dzisq;
+        } else {
+            constexpr int iz = (S + 2) / 3, t = S + 2 - 3 * iz;
+            if constexpr (iz < NZC) {
+                if constexpr (t == 2) return wkd(std::integral_constant<int, iz>{});
+                else return wka(std::integral_constant<int, iz>{});
+            } else {
+                const T w2 = wkd(std::integral_constant<int, NZC - 2>{});
+                const T d1 = wkd(std::integral_constant<int, NZC - 1>{});
+                if constexpr (t == 2) return T(2) * d1 - w2;
+                else return wka(std::integral_constant<int, NZC - 1>{}) + dz2 * (T(3) * d1 - w2);
+            }
+        }
     }
 };
 
@@ -853,7 +966,7 @@ struct Engine {
     }
     // fftw_inv_comb_fwd over nslab slab groups (transposed output); bytes = the algorithmic bytes of
     // the launch, for the profiling mode that times this site.
-    // 4-wave blocks: the kernel runs three waves per SIMD, so three blocks share a CU
+    // 4-wave blocks: in FP64 the kernel runs three waves per SIMD, so three blocks share a CU
     static constexpr int NWC = 4;
     template <int NN, class Pro, class Comb>
     void inv_comb_fwd_wave(int nslab, Pro pro, Comb comb, T* out, double bytes, int site) {
@@ -1033,7 +1146,15 @@ struct Engine {
     void coeffs(Fields f, T* sigx, T* sigy, T* cpt1, T* cpt2);
     void cpsource(Fields f, T* sp0);
     T* psolve_source_wave(const T* ps, const T* sp0, const T* sigx, const T* sigy, const T* cpt1, const T* cpt2);
-    bool psolve_solve_wave(const T* SPt, T* SP, T* ps, double* sums);
+    // the fused iteration of the wave path with a compile-time nz: derivative rows in the tiled layout
+    // (tile_pos, TILE_RB = the rows of one source-fold block), written by the column-solve launch and
+    // read by the source fold.  (64-byte runs in FP32 as well, 16-row tiles, made the writer faster and
+    // the fold's strided two-row loads slower by as much.)
+    static constexpr int TILE_RB = 2 * NWC;
+    bool psolve_fused() const { return wave_n() && nz_spec(); }
+    void psolve_deriv_tiled(const T* ps, T* D);
+    T* psolve_fold_tiled(const T* D, const T* sp0, const T* sigx, const T* sigy, const T* cpt1, const T* cpt2);
+    void psolve_solve_fused(const T* SPt, T* SP, T* ps, T* D);
     void psolve_iterate_device(const T* sp0, const T* sigx, const T* sigy, const T* cpt1, const T* cpt2, T* SP, T* ps,
                                const T* pn, int nitmax, int* iters, double* errp, int* status);
     int psolve_step(const T* sp0, const T* sigx, const T* sigy, const T* cpt1, const T* cpt2, T* SP, T* ps, double* sums);

@@ -47,6 +47,16 @@ __device__ __forceinline__ void wave_lds_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// f(integral_constant<int, I>) for I = B .. E - 1, in order (a loop whose index must be a constant
+// expression, e.g. to pick compile-time register indices)
+template <int B, int E, class F>
+__device__ __forceinline__ void static_for(F&& f) {
+    if constexpr (B < E) {
+        f(std::integral_constant<int, B>{});
+        static_for<B + 1, E>(f);
+    }
+}
+
 // Row stride of the LDS stage [f][row] through which a block of NW waves writes (or reads) 2 NW rows
 // transposed: one pad value per f, so that the lanes of a wave, which access the stage at an odd
 // stride of 2 NW + 1 values, spread over the banks instead of all hitting one, and the stage fits in
@@ -377,9 +387,13 @@ __global__ void __launch_bounds__(64 * NW) fftw_inv_pass(const C2<T>* __restrict
 // starts from comb.init(s, r, j); the accumulated real row is transformed forward again and stored
 // transposed: element (r, f) at out[r + n f + n n s].  Nothing between the two transforms goes
 // through HBM.
-template <typename T, int N, int NW, class Pro, class Comb>
-__global__ void __launch_bounds__(64 * NW) fftw_inv_comb_fwd(const C2<T>* __restrict__ tw, int nslab, Pro pro, Comb comb, T* tout,
-                                                            Gate gate) {
+//
+// TILED: the inputs are read in the tiled layout instead (tile_pos with RB = Pro::RB, a multiple of 2 NW,
+// so that the block's rows lie in one tile): pro.tile(s, k, r0) points at row r0, position 0 of input k,
+// and each wave's two rows are adjacent at every position f (one 2-value load each).
+template <typename T, int N, int NW, class Pro, class Comb, bool TILED = false>
+__global__ void __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 5 : 3)))
+fftw_inv_comb_fwd(const C2<T>* __restrict__ tw, int nslab, Pro pro, Comb comb, T* tout, Gate gate) {
     constexpr int E = WaveShape<N>::E, BPS = N / (2 * NW), NT = E / 2 + 1;
     extern __shared__ __align__(16) unsigned char lds_raw[];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
@@ -408,15 +422,44 @@ __global__ void __launch_bounds__(64 * NW) fftw_inv_comb_fwd(const C2<T>* __rest
         // an input that is the derivative along the transform axis of the previous one is packed from
         // the previous input's registers instead of being fetched
         const bool dcur = k > 0 && comb.deriv_of_prev(s, k);
-        if (!dcur) wave_fetch_herm<N>(h, ln, [&](int hh, int f) { return pro(s, k, ra + hh, f); });
+        if constexpr (TILED) {
+            if (!dcur) {
+                // the wave's two rows are adjacent in every tile row f: one 2-value load per position
+                constexpr int RB = Pro::RB;
+                static_assert(RB % (2 * NW) == 0, "a block's rows lie in one tile");
+                const C2<T>* __restrict__ src = reinterpret_cast<const C2<T>*>(pro.tile(s, k, r0) + 2 * wv);
+#pragma unroll
+                for (int t = 0; t < NT; t++) {
+                    const int kk = ln + 64 * t;
+                    if (t < NT - 1 || ln == 0) {
+                        const bool has_s = kk > 0 && 2 * kk < N;
+                        const C2<T> c = src[(size_t)kk * (RB / 2)];
+                        const C2<T> sn = has_s ? src[(size_t)(N - kk) * (RB / 2)] : C2<T>{T(0), T(0)};
+                        h.ca[t] = c.x;
+                        h.cb[t] = c.y;
+                        h.sa[t] = sn.x;
+                        h.sb[t] = sn.y;
+                    }
+                }
+            }
+        } else {
+            if (!dcur) wave_fetch_herm<N>(h, ln, [&](int hh, int f) { return pro(s, k, ra + hh, f); });
+        }
         if (dcur) wave_pack_herm<N, true>(v, buf, ln, h);
         else wave_pack_herm<N>(v, buf, ln, h);
         wave_stockham<N, true>(v, buf, tw, ln);
+        // every coefficient load of the input issued before the first is used
+        T cx[E], cy[E];
 #pragma unroll
         for (int r = 0; r < E; r++) {
             const int j = ln + 64 * r;
-            acc[r].x = acc[r].x + comb.coef(s, k, ra, j) * (v[r].x * sc);
-            acc[r].y = acc[r].y + comb.coef(s, k, ra + 1, j) * (v[r].y * sc);
+            cx[r] = comb.coef(s, k, ra, j);
+            cy[r] = comb.coef(s, k, ra + 1, j);
+        }
+#pragma unroll
+        for (int r = 0; r < E; r++) {
+            acc[r].x = acc[r].x + cx[r] * (v[r].x * sc);
+            acc[r].y = acc[r].y + cy[r] * (v[r].y * sc);
         }
     }
     int lp = lane;
@@ -574,71 +617,171 @@ __global__ void __launch_bounds__(64 * NW) fftw_deriv_lines(const C2<T>* __restr
     }
 }
 
-// Forward pass along x of every level of one row, then a per-column functor across the levels.  Block
-// b = mb N + ky (member mb of a batch) owns row ky of the member's LV slabs ld(mb, l, ky, x) (l < LV); its
-// NWV waves transform the rows as level pairs (pair m on wave m mod NWV), the spectra land in an LDS
-// column store cs[l N + kx], and each thread then calls col(mb, kx, ky, cs + kx) for its columns (level
-// l at offset l N).  The Acc<2> results are summed over the block, in a fixed order, into part[2 b + c].
-// The block does nothing once its member's gate flag is set.
-template <typename T, int N, int NWV, int LV, class Ld, class Col>
-__global__ void __launch_bounds__(64 * NWV) fftw_fwd_rows_cols(const C2<T>* __restrict__ tw, Ld ld, Col col,
-                                                              double* part, Gate gate) {
+// Levels of fftw_rows_cols_inv's column store that have LDS of their own: those beyond 2 NWV
+template <int NWV, int LV>
+constexpr int rows_cols_x() { return LV > 2 * NWV ? LV - 2 * NWV : 0; }
+template <int N, int NWV, int LV, typename T>
+constexpr size_t rows_cols_inv_lds_bytes() {
+    constexpr int X = rows_cols_x<NWV, LV>();
+    size_t ex = (size_t)NWV * wave_region<N>() * sizeof(C2<T>);
+    size_t cs = (size_t)(LV - X) * N * sizeof(T);
+    return (size_t)X * N * sizeof(T) + (ex > cs ? ex : cs);
+}
+
+// Position of element (x, f) of a slab in the tiled layout that the column-solve launch below writes
+// and the staged inputs of fftw_inv_comb_fwd read: tiles of RB consecutive x (the rows of one
+// transposing block of that launch) by all N values of f, each tile contiguous with x fastest, so that
+// a block reads its tile as one contiguous run and a row f of the writer stores runs of RB values.
+template <int N, int RB>
+__device__ __forceinline__ size_t tile_pos(int x, int f) {
+    return (size_t)(x / RB) * (RB * N) + (size_t)f * RB + (x % RB);
+}
+
+// Forward pass along x of every level of one row, a per-column functor across the levels, then, in the
+// same launch, inverse transforms along kx of ND rows derived from the columns.  Block b = mb N + ky
+// (member mb of a batch) does nothing once its member's gate flag is set.  Its NWV waves transform the LV
+// levels of row ky (ld(mb, l, ky, x)) as level pairs into an LDS column store cs[l N + kx]; then
+// col(mb, kx, ky, cs + kx, pc) runs per column (at most one per thread) and leaves the column's new values
+// in the registers pc[LV].  Then, in rounds of NWV row pairs, every thread writes the raw values
+// der.raw<s>(pc) of the round's rows s at its column into the exchange region of the wave that owns the
+// row's pair, and each wave packs its pair as Hermitian rows (a row with der.xd(s) takes the derivative
+// along kx on the way: d[f] = dmul(f) raw[mir(f)]), transforms it inverse along kx and stores element
+// (x, ky) of row s at dout[(mb ND + s) N N + tile_pos<N, RB>(x, ky)].  The Acc<2> results of col are
+// summed per block, in a fixed order, into part[2 b + c].  Nothing between the column solve and the
+// derived rows goes through HBM.
+//
+// SOLVE = false: no forward pass and no column functor; col.load(mb, kx, ky, pc) fills each column's
+// registers (from HBM) and nothing is written to part (the derived rows of a given column field).
+template <typename T, int N, int NWV, int LV, int ND, int RB, class Ld, class Col, class Der, bool SOLVE = true>
+__global__ void __launch_bounds__(64 * NWV) __attribute__((amdgpu_waves_per_eu(4)))
+fftw_rows_cols_inv(const C2<T>* __restrict__ tw, Ld ld, Col col, Der der, T* __restrict__ dout, double* part, Gate gate) {
     const int mb = blockIdx.x / N, ky = blockIdx.x - mb * N;
     if (gate.member_off(mb)) return;
-    constexpr int E = WaveShape<N>::E, NT = E / 2 + 1, NP = (LV + 1) / 2, MAXP = (NP + NWV - 1) / NWV;
+    constexpr int E = WaveShape<N>::E, NT = E / 2 + 1;
+    // levels [0, X) of the column store have LDS of their own ahead of the exchange regions and receive
+    // their spectra as soon as they are transformed (as XP pairs, the first a single level when X is odd);
+    // levels [X, LV), at most 2 NWV, overlay the regions and wait in registers, one pair per wave, until
+    // every wave is done with its region
+    constexpr int X = SOLVE ? rows_cols_x<NWV, LV>() : 0, XP = (X + 1) / 2, MP = (LV - X + 1) / 2;
+    constexpr int NDP = (ND + 1) / 2, ROUNDS = (NDP + NWV - 1) / NWV;
+    static_assert(N <= 64 * NWV, "one column per thread");
+    static_assert(!SOLVE || MP <= NWV, "one pair of the overlaid levels per wave");
     extern __shared__ __align__(16) unsigned char lds_raw[];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    C2<T>* buf = reinterpret_cast<C2<T>*>(lds_raw) + wv * wave_region<N>();
+    C2<T>* const regions = reinterpret_cast<C2<T>*>(lds_raw + (size_t)X * N * sizeof(T));
+    C2<T>* buf = regions + wv * wave_region<N>();
     T* cs = reinterpret_cast<T*>(lds_raw);
 
-    T ar[MAXP][NT], ai[MAXP][NT], br[MAXP][NT], bi[MAXP][NT];
+    // transform the pair of levels (l0, l1) (l1 < 0: l0 alone) of row ky along x
+    auto fwd_pair = [&](int l0, int l1, T (&ar)[NT], T (&ai)[NT], T (&br)[NT], T (&bi)[NT]) {
+        C2<T> v[E];
 #pragma unroll
-    for (int m = 0; m < MAXP; m++) {
-        const int pr = wv + m * NWV;  // wave-uniform
-        if (pr < NP) {
-            const int l0 = 2 * pr, l1 = 2 * pr + 1;
-            C2<T> v[E];
-#pragma unroll
-            for (int r = 0; r < E; r++) {
-                const int x = lane + 64 * r;
-                v[r] = C2<T>{ld(mb, l0, ky, x), l1 < LV ? ld(mb, l1, ky, x) : T(0)};
-            }
-            wave_stockham<N, false>(v, buf, tw, lane);
-            wave_split_pair<N>(v, buf, lane, ar[m], ai[m], br[m], bi[m]);
+        for (int r = 0; r < E; r++) {
+            const int x = lane + 64 * r;
+            v[r] = C2<T>{ld(mb, l0, ky, x), l1 >= 0 ? ld(mb, l1, ky, x) : T(0)};
         }
-    }
-    __syncthreads();  // every wave is done with its exchange region (the column store overlays them)
+        wave_stockham<N, false>(v, buf, tw, lane);
+        wave_split_pair<N>(v, buf, lane, ar, ai, br, bi);
+    };
+    auto put_pair = [&](int l0, int l1, const T (&ar)[NT], const T (&ai)[NT], const T (&br)[NT], const T (&bi)[NT]) {
 #pragma unroll
-    for (int m = 0; m < MAXP; m++) {
-        const int pr = wv + m * NWV;
-        if (pr < NP) {
-            const int l0 = 2 * pr, l1 = 2 * pr + 1;
+        for (int t = 0; t < NT; t++) {
+            const int k = lane + 64 * t;
+            if (t < NT - 1 || lane == 0) {
+                cs[l0 * N + k] = ar[t];
+                if (l1 >= 0) cs[l1 * N + k] = br[t];
+                if (k > 0 && 2 * k < N) {
+                    cs[l0 * N + N - k] = ai[t];
+                    if (l1 >= 0) cs[l1 * N + N - k] = bi[t];
+                }
+            }
+        }
+    };
+    if constexpr (SOLVE) {
+        T ar[NT], ai[NT], br[NT], bi[NT];
+        for (int j = wv; j < XP; j += NWV) {  // wave-uniform
+            const int l0 = (X & 1) ? (j == 0 ? 0 : 2 * j - 1) : 2 * j, l1 = (X & 1) && j == 0 ? -1 : l0 + 1;
+            fwd_pair(l0, l1, ar, ai, br, bi);
+            put_pair(l0, l1, ar, ai, br, bi);
+        }
+        const int l0 = X + 2 * wv, l1 = l0 + 1 < LV ? l0 + 1 : -1;
+        if (wv < MP) fwd_pair(l0, l1, ar, ai, br, bi);
+        __syncthreads();  // every wave is done with its exchange region (the overlaid levels cover them)
+        if (wv < MP) put_pair(l0, l1, ar, ai, br, bi);
+        __syncthreads();
+    }
+    const int kx = threadIdx.x;
+    const bool has_col = kx < N;
+    T pc[LV];
+    double s0 = 0.0, s1 = 0.0;
+    if constexpr (SOLVE) {
+        if (has_col) {
+            const auto a = col(mb, kx, ky, cs + kx, pc);
+            s0 = a.v[0];
+            s1 = a.v[1];
+        }
+        for (int off = 32; off > 0; off >>= 1) {
+            s0 += __shfl_down(s0, off, 64);
+            s1 += __shfl_down(s1, off, 64);
+        }
+    } else {
+        if (has_col) col.load(mb, kx, ky, pc);
+    }
+    const T sc = T(1) / sqrt(T(N));
+    T* const db = dout + (size_t)mb * ND * N * N;
+    static_for<0, ROUNDS>([&](auto qc) {
+        constexpr int q = decltype(qc)::value;
+        __syncthreads();  // the column store, or the previous round's regions, are no longer read
+        if (has_col) {
+            static_for<0, NWV>([&](auto wc) {
+                constexpr int pi = q * NWV + decltype(wc)::value, sa = 2 * pi, sb = 2 * pi + 1;
+                if constexpr (pi < NDP) {
+                    C2<T>* reg = regions + decltype(wc)::value * wave_region<N>();
+                    T vb = T(0);
+                    if constexpr (sb < ND) vb = der.template raw<sb>(pc);
+                    reg[wpad(kx)] = C2<T>{der.template raw<sa>(pc), vb};
+                }
+            });
+        }
+        __syncthreads();
+        const int pi = q * NWV + wv;  // wave-uniform
+        if (pi < NDP) {
+            const int sa = 2 * pi, sb = 2 * pi + 1;
+            const bool xa = der.xd(sa), xb = sb < ND && der.xd(sb);
+            // pack in place: the lane that reads positions k and N - k (k = lane + 64 t <= N/2) is the
+            // only one that writes them
 #pragma unroll
             for (int t = 0; t < NT; t++) {
                 const int k = lane + 64 * t;
                 if (t < NT - 1 || lane == 0) {
-                    cs[l0 * N + k] = ar[m][t];
-                    if (l1 < LV) cs[l1 * N + k] = br[m][t];
-                    if (k > 0 && 2 * k < N) {
-                        cs[l0 * N + N - k] = ai[m][t];
-                        if (l1 < LV) cs[l1 * N + N - k] = bi[m][t];
-                    }
+                    const bool has_s = k > 0 && 2 * k < N;
+                    const C2<T> c = buf[wpad(k)];
+                    const C2<T> sn = has_s ? buf[wpad(N - k)] : C2<T>{T(0), T(0)};
+                    const T m = (k == 0 || 2 * k == N) ? T(0) : T(k);
+                    // derivative along kx: -k times the sine part at k, k times the cosine part at N - k
+                    const T ca = xa ? -m * sn.x : c.x, sa_ = xa ? m * c.x : sn.x;
+                    const T cb = xb ? -m * sn.y : c.y, sb_ = xb ? m * c.y : sn.y;
+                    buf[wpad(k)] = C2<T>{ca - sb_, sa_ + cb};
+                    if (has_s) buf[wpad(N - k)] = C2<T>{ca + sb_, cb - sa_};
                 }
             }
+            wave_lds_sync();
+            C2<T> v[E];
+#pragma unroll
+            for (int r = 0; r < E; r++) v[r] = buf[wpad(lane + 64 * r)];
+            wave_stockham<N, true>(v, buf, tw, lane);
+            T* oa = db + (size_t)sa * N * N;
+#pragma unroll
+            for (int r = 0; r < E; r++) {
+                const int x = lane + 64 * r;
+                const size_t o = tile_pos<N, RB>(x, ky);
+                oa[o] = v[r].x * sc;
+                if (sb < ND) oa[(size_t)N * N + o] = v[r].y * sc;
+            }
         }
-    }
-    __syncthreads();
-    double s0 = 0.0, s1 = 0.0;
-    for (int kx = threadIdx.x; kx < N; kx += 64 * NWV) {
-        const auto a = col(mb, kx, ky, cs + kx);
-        s0 += a.v[0];
-        s1 += a.v[1];
-    }
-    for (int off = 32; off > 0; off >>= 1) {
-        s0 += __shfl_down(s0, off, 64);
-        s1 += __shfl_down(s1, off, 64);
-    }
-    __syncthreads();  // the column store is no longer read
+    });
+    if constexpr (!SOLVE) return;
+    __syncthreads();  // the exchange regions are no longer read
     double* red = reinterpret_cast<double*>(lds_raw);
     if (lane == 0) red[2 * wv] = s0, red[2 * wv + 1] = s1;
     __syncthreads();
@@ -650,11 +793,5 @@ __global__ void __launch_bounds__(64 * NWV) fftw_fwd_rows_cols(const C2<T>* __re
     }
 }
 
-template <int N, int NWV, int LV, typename T>
-constexpr size_t rows_cols_lds_bytes() {
-    size_t ex = (size_t)NWV * wave_region<N>() * sizeof(C2<T>);
-    size_t cs = (size_t)LV * N * sizeof(T);
-    return ex > cs ? ex : cs;
-}
 
 }  // namespace sw

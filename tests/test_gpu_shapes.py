@@ -9,7 +9,8 @@ zeta = f/2, delta = gamma = 0, unless stated):
 * 256x8, 100 steps: the FP64 comparison of the second configuration, with the psolve iteration
   counts of every step;
 * 1536x24, one step: the FP64 leg of the large mixed-radix configuration (seeded k^-3 PV plus a
-  random divergence of rms 0.01 f), on the block-per-rows transforms.
+  random divergence of rms 0.01 f), on the block-per-rows transforms; the oracle's sensitivity at this
+  size is a recorded measurement (SENS_1536x24), so that the oracle runs once.
 
 Each test prints its per-field relative L2 errors.  One-step bounds: the rounding of any transform
 other than the reference's own perturbs the step at ~1e-16, and the step amplifies such noise by a
@@ -125,8 +126,18 @@ def test_100_steps_256x8(sw, oracle):
         assert errs[f] < LONG_TOL[f], f
 
 
+# The oracle's sensitivity at 1536x24 (seed 2, rms delta 0.01 f), measured as one_step measures it (its
+# step re-run with 1e-16 relative noise in qs; 16 OpenMP threads need ~2 minutes per oracle step at this
+# size, so the test runs the oracle once and takes these figures as the conditioning of the step)
+SENS_1536x24 = {"ri": 3.6e-16, "r": 3.0e-15, "u": 7.7e-16, "v": 1.1e-15, "w": 2.0e-12, "zeta": 2.1e-15,
+                "z": 1.2e-16, "zx": 5.2e-13, "zy": 7.1e-13, "ps": 4.0e-14, "pn": 3.4e-14, "dpn": 1.2e-12,
+                "aa": 3.5e-12, "qs": 2.0e-15, "ds": 2.3e-12, "gs": 6.0e-11}
+
+
 @pytest.mark.timeout(900)
 def test_one_step_1536x24(sw, oracle):
     ng, nz = 1536, 24
-    errs, sens = one_step(sw, oracle, ng, nz, seed=2, delta_frac=0.01)
-    assert_within_conditioning(errs, sens)
+    errs, _ = one_step(sw, oracle, ng, nz, sensitivity=False, seed=2, delta_frac=0.01)
+    print("1536x24 GPU error / oracle sensitivity: " + " ".join(f"{f}={errs[f] / SENS_1536x24[f]:.2f}" for f in ALL),
+          flush=True)
+    assert_within_conditioning(errs, SENS_1536x24)
