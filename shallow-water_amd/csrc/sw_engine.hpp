@@ -344,13 +344,18 @@ struct PsolveCol {
     // the same, leaving the column's new ps (levels 0 .. NZC - 1) and its top level NZC (which the
     // solve does not change) in pc
     __device__ __forceinline__ Acc<2> operator()(int mb, int kx, int ky, const T* c, T (&pc)[NZC + 1]) const {
+        const Acc<2> r = solve(mb, kx, ky, [&](int iz) { return (double)c[iz * n]; }, pc);
+        sptop0[(size_t)mb * mstride + kx + (size_t)n * ky] = c[NZC * n];
+        return r;
+    }
+    // the solve alone, S(iz) giving the source at level iz (in LDS or in HBM)
+    template <class SF>
+    __device__ __forceinline__ Acc<2> solve(int mb, int kx, int ky, SF S, T (&pc)[NZC + 1]) const {
         T* ps = ps0 + (size_t)mb * mstride;
-        T* sptop = sptop0 + (size_t)mb * mstride;
         const size_t p = kx + (size_t)n * ky;
         const double wgt = ((kx == 0 || 2 * kx == n) ? 1.0 : 2.0) * ((ky == 0 || 2 * ky == n) ? 1.0 : 2.0);
         const double rx = 2 * kx <= n ? kx : n - kx, ry = 2 * ky <= n ? ky : n - ky;
         const PressureLU lu(rx * rx + ry * ry, dzisq, (double)filt[p]);
-        auto S = [&](int iz) { return (double)c[iz * n]; };
         double colr[NZC], er[NZC];
         double h = lu.h0();
         er[0] = lu.e0(h);
@@ -378,7 +383,6 @@ struct PsolveCol {
             pc[iz] = neu;
         }
         pc[NZC] = ps[(size_t)NZC * nn + p];
-        sptop[p] = c[NZC * n];
         Acc<2> r;
         r.v[0] = wgt * dsum;
         r.v[1] = wgt * osum;
