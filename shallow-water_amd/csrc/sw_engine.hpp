@@ -798,9 +798,41 @@ struct Engine {
         __device__ void operator()(int, int, int, T) const {}
     };
 
-    // Transform lengths with wave-per-sequence kernels (sw_fft_wave.hpp); others use the
-    // block-per-rows kernels of sw_fft.hpp.
+    // Transform lengths with wave-per-sequence kernels (sw_fft_wave.hpp); 1536 = 3 x 512 with
+    // three-wave groups for its 2-D transforms (fwd2d, inv2d); others use the block-per-rows kernels of
+    // sw_fft.hpp.
     bool wave_n() const { return ng == 128 || ng == 256 || ng == 512; }
+    bool wave3_n() const { return ng == 1536; }
+    // groups per block of the three-wave transposing passes: 8 rows, runs of 8 values per output row
+    static constexpr int NG3 = 4;
+    template <class Pro, class Epi>
+    void fwd2d_wave3(int nslab, Pro pro, Epi epi, T* t) {
+        constexpr int N = 512, N3 = 1536;
+        const size_t l1 = wave3_lds_bytes<N, NG3, T>(), l2 = wave3_lds_bytes<N, 1, T>();
+        lds_optin((const void*)fftw3_fwd_pass<T, N, NG3, true, Pro, NoEpi>, l1);
+        lds_optin((const void*)fftw3_fwd_pass<T, N, 1, false, LoadPlain<T>, Epi>, l2);
+        hipLaunchKernelGGL((fftw3_fwd_pass<T, N, NG3, true, Pro, NoEpi>), dim3(nslab * (N3 / (2 * NG3))), dim3(192 * NG3), l1, st,
+                           plan.tw, nslab, pro, NoEpi{}, t, gate_);
+        SW_HIP(hipGetLastError());
+        LoadPlain<T> lp{t, n2, ng};
+        hipLaunchKernelGGL((fftw3_fwd_pass<T, N, 1, false, LoadPlain<T>, Epi>), dim3(nslab * (N3 / 2)), dim3(192), l2, st, plan.tw,
+                           nslab, lp, epi, (T*)nullptr, gate_);
+        SW_HIP(hipGetLastError());
+    }
+    template <class Pro, class Epi>
+    void inv2d_wave3(int nslab, Pro pro, Epi epi, T* t) {
+        constexpr int N = 512, N3 = 1536;
+        const size_t l1 = wave3_lds_bytes<N, NG3, T>(), l2 = wave3_lds_bytes<N, 1, T>();
+        lds_optin((const void*)fftw3_inv_pass<T, N, NG3, true, Pro, NoEpi>, l1);
+        lds_optin((const void*)fftw3_inv_pass<T, N, 1, false, LoadPlain<T>, Epi>, l2);
+        hipLaunchKernelGGL((fftw3_inv_pass<T, N, NG3, true, Pro, NoEpi>), dim3(nslab * (N3 / (2 * NG3))), dim3(192 * NG3), l1, st,
+                           plan.tw, nslab, pro, NoEpi{}, t, gate_);
+        SW_HIP(hipGetLastError());
+        LoadPlain<T> lp{t, n2, ng};
+        hipLaunchKernelGGL((fftw3_inv_pass<T, N, 1, false, LoadPlain<T>, Epi>), dim3(nslab * (N3 / 2)), dim3(192), l2, st, plan.tw,
+                           nslab, lp, epi, (T*)nullptr, gate_);
+        SW_HIP(hipGetLastError());
+    }
     template <class F>
     void wave_dispatch(F f) {
         if (ng == 512) f(std::integral_constant<int, 512>{});
@@ -846,6 +878,11 @@ struct Engine {
     template <int NW = 0, class Pro, class Epi>
     void fwd2d(int nslab, Pro pro, Epi epi, int site = SITE_OTHER, T* inter = nullptr) {
         if (nslab <= 0) return;
+        if (wave3_n()) {
+            T* t = inter ? inter : tmp_for(nslab);
+            fwd2d_wave3(nslab, pro, epi, t);
+            return;
+        }
         if (wave_n()) {
             T* t = inter ? inter : tmp_for(nslab);
             wave_dispatch([&](auto nn) {
@@ -880,6 +917,10 @@ struct Engine {
     template <int NW = 0, class Pro, class Epi>
     void inv2d(int nslab, Pro pro, Epi epi, int site = SITE_OTHER) {
         if (nslab <= 0) return;
+        if (wave3_n()) {
+            inv2d_wave3(nslab, pro, epi, tmp_for(nslab));
+            return;
+        }
         if (wave_n()) {
             T* t = tmp_for(nslab);
             wave_dispatch([&](auto nn) {

@@ -75,7 +75,8 @@ constexpr size_t wave_lds_bytes() {
 
 // P radix-E Stockham passes over the sequence held as v[r] = x[lane + 64 r]; afterwards
 // v[r] = X[lane + 64 r].  buf is this wave's LDS region.  Every thread of the block calls this.
-template <int N, bool INV, typename T>
+// TWS: the table holds exp(-2 pi i k / (TWS N)) (a longer transform's table, every TWS-th entry used).
+template <int N, bool INV, typename T, int TWS = 1>
 __device__ __forceinline__ void wave_stockham(C2<T> (&v)[WaveShape<N>::E], C2<T>* buf, const C2<T>* __restrict__ tw,
                                               int lane) {
     constexpr int E = WaveShape<N>::E, P = WaveShape<N>::P;
@@ -92,7 +93,7 @@ __device__ __forceinline__ void wave_stockham(C2<T> (&v)[WaveShape<N>::E], C2<T>
             const int stride = N / (Ns * E);
 #pragma unroll
             for (int r = 1; r < E; r++) {
-                C2<T> w = twp[r * k * stride];
+                C2<T> w = twp[r * k * stride * TWS];
                 if (INV) w.y = -w.y;
                 v[r] = cmul(v[r], w);
             }
@@ -793,5 +794,211 @@ fftw_rows_cols_inv(const C2<T>* __restrict__ tw, Ld ld, Col col, Der der, T* __r
     }
 }
 
+
+// ---------------------------------------------------------------- length 3 N (N = 512: 1536)
+// A complex sequence of N3 = 3 N values is transformed by a group of three waves: wave j of the group
+// transforms the decimated subsequence z[3 m + j] (m = lane + 64 r) with wave_stockham<N> (the N3 table,
+// every third entry), twiddles it by W^(j k) (W = exp(-+2 pi i / N3)), and the three are combined by a
+// radix-3 butterfly through the group's LDS (its three exchange regions, which also hold the whole
+// sequence when it is exchanged: element n at region n / N, slot wpad(n mod N)).  A block holds NG
+// groups, each owning a pair of real rows, as the wave-per-sequence passes above.
+template <int N>
+__device__ __forceinline__ int g3pos(int n) {
+    return (n / N) * wave_region<N>() + wpad(n % N);
+}
+template <int N, int NG, typename T>
+constexpr size_t wave3_lds_bytes() {
+    size_t ex = (size_t)NG * 3 * wave_region<N>() * sizeof(C2<T>);
+    size_t stg = NG > 1 ? (size_t)3 * N * stage_stride<NG>() * sizeof(T) : 0;
+    return ex > stg ? ex : stg;
+}
+
+// v[r] = z[3 (lane + 64 r) + wj] on entry; Z[wj N + lane + 64 r] on exit (INV: the inverse transform).
+// grp: the group's three regions; no wave of the group may still read them on entry.  Every thread of
+// the block calls this (it holds block barriers).
+template <int N, bool INV, typename T>
+__device__ __forceinline__ void wave3_dit(C2<T> (&v)[WaveShape<N>::E], C2<T>* grp, int wj, const C2<T>* __restrict__ tw3,
+                                          int lane) {
+    constexpr int E = WaveShape<N>::E;
+    C2<T>* reg = grp + wj * wave_region<N>();
+    wave_stockham<N, INV, T, 3>(v, reg, tw3, lane);
+    const C2<T>* twp = tw3;
+    asm volatile("" : "+s"(twp));
+    if (wj > 0) {
+#pragma unroll
+        for (int r = 0; r < E; r++) {
+            C2<T> w = twp[wj * (lane + 64 * r)];
+            if (INV) w.y = -w.y;
+            v[r] = cmul(v[r], w);
+        }
+    }
+    wave_lds_sync();  // the transform's last reads of reg are done
+#pragma unroll
+    for (int r = 0; r < E; r++) reg[wpad(lane + 64 * r)] = v[r];
+    __syncthreads();
+    // Z[k + N q] = b0 + w3^q b1 + w3^(2 q) b2, w3 = exp(-+2 pi i / 3) = c + i sn
+    const T c = T(-0.5), sn = INV ? T(0.86602540378443864676) : T(-0.86602540378443864676);
+#pragma unroll
+    for (int r = 0; r < E; r++) {
+        const int k = wpad(lane + 64 * r);
+        const C2<T> b0 = grp[k], b1 = grp[wave_region<N>() + k], b2 = grp[2 * wave_region<N>() + k];
+        const C2<T> t1 = cadd(b1, b2), t2 = csub(b1, b2);
+        if (wj == 0) {
+            v[r] = cadd(b0, t1);
+        } else {
+            // w3 b1 + w3^2 b2 = c t1 + i sn t2; w3^2 b1 + w3 b2 = c t1 - i sn t2
+            const T sg = wj == 1 ? sn : -sn;
+            v[r] = C2<T>{b0.x + c * t1.x - sg * t2.y, b0.y + c * t1.y + sg * t2.x};
+        }
+    }
+    __syncthreads();  // every wave has read the regions
+}
+
+// Forward (real -> Hermitian) pass of length N3 over rows of N3 x N3 slabs, as fftw_fwd_pass: group g of
+// the block owns rows ra = r0 + 2 g and ra + 1 (pro(s, r, j)); TRANS: element (r, f) to
+// tout[r + N3 f + N3 N3 s] through an LDS stage (runs of 2 NG values per output row); otherwise epi(s, r,
+// f, value).  A gated slab group (gate.off(s)) does nothing.
+template <typename T, int N, int NG, bool TRANS, class Pro, class Epi>
+__global__ void __launch_bounds__(192 * NG) fftw3_fwd_pass(const C2<T>* __restrict__ tw3, int nslab, Pro pro, Epi epi,
+                                                          T* tout, Gate gate) {
+    constexpr int E = WaveShape<N>::E, N3 = 3 * N, BPS = N3 / (2 * NG), HT = E / 2;
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, g = wv / 3, wj = wv - 3 * g;
+    C2<T>* grp = reinterpret_cast<C2<T>*>(lds_raw) + g * 3 * wave_region<N>();
+    const int s = blockIdx.x / BPS, r0 = (blockIdx.x - s * BPS) * 2 * NG, ra = r0 + 2 * g;
+    if (s >= nslab || gate.off(s)) return;
+
+    C2<T> v[E];
+#pragma unroll
+    for (int r = 0; r < E; r++) {
+        const int j = 3 * (lane + 64 * r) + wj;
+        v[r] = C2<T>{pro(s, ra, j), pro(s, ra + 1, j)};
+    }
+    wave3_dit<N, false>(v, grp, wj, tw3, lane);
+#pragma unroll
+    for (int r = 0; r < E; r++) grp[wj * wave_region<N>() + wpad(lane + 64 * r)] = v[r];
+    __syncthreads();
+    // split the two real transforms at k = wj N/2 + lane + 64 t (and N3/2 on wave 2, lane 0)
+    const T sc = T(0.5) / sqrt(T(N3));
+    T ar[HT + 1], ai[HT + 1], br[HT + 1], bi[HT + 1];
+#pragma unroll
+    for (int t = 0; t <= HT; t++) {
+        const int k = wj * (N / 2) + lane + 64 * t;
+        if (t < HT || (wj == 2 && lane == 0)) {
+            const int kc = k == 0 ? 0 : N3 - k;
+            const C2<T> z = grp[g3pos<N>(k)], w = grp[g3pos<N>(kc)];
+            ar[t] = (z.x + w.x) * sc;
+            ai[t] = (z.y - w.y) * sc;
+            br[t] = (z.y + w.y) * sc;
+            bi[t] = (w.x - z.x) * sc;
+        }
+    }
+    if constexpr (!TRANS) {
+#pragma unroll
+        for (int t = 0; t <= HT; t++) {
+            const int k = wj * (N / 2) + lane + 64 * t;
+            if (t < HT || (wj == 2 && lane == 0)) {
+                epi(s, ra, k, ar[t]);
+                epi(s, ra + 1, k, br[t]);
+                if (k > 0 && 2 * k < N3) {
+                    epi(s, ra, N3 - k, ai[t]);
+                    epi(s, ra + 1, N3 - k, bi[t]);
+                }
+            }
+        }
+    } else {
+        static_assert(!TRANS || NG > 1, "a transposed store stages 2 NG rows through the block's LDS");
+        constexpr int RB = 2 * NG, RS = stage_stride<NG>();
+        T* stg = reinterpret_cast<T*>(lds_raw);  // [f][row], row stride RS
+        __syncthreads();                         // every group is done with its regions
+#pragma unroll
+        for (int t = 0; t <= HT; t++) {
+            const int k = wj * (N / 2) + lane + 64 * t;
+            if (t < HT || (wj == 2 && lane == 0)) {
+                stg[k * RS + 2 * g] = ar[t];
+                stg[k * RS + 2 * g + 1] = br[t];
+                if (k > 0 && 2 * k < N3) {
+                    stg[(N3 - k) * RS + 2 * g] = ai[t];
+                    stg[(N3 - k) * RS + 2 * g + 1] = bi[t];
+                }
+            }
+        }
+        __syncthreads();
+        T* o = tout + (size_t)s * N3 * N3 + r0;
+#pragma unroll 4
+        for (int e = threadIdx.x; e < RB * N3; e += 192 * NG) {
+            const int f = e / RB, rr = e - f * RB;
+            o[(size_t)f * N3 + rr] = stg[f * RS + rr];
+        }
+    }
+}
+
+// Inverse (Hermitian -> real) pass of length N3, as fftw_inv_pass: pro(s, r, f) gives Hermitian row r at
+// packed position f; TRANS: element (r, x) to tout[r + N3 x + N3 N3 s]; otherwise epi(s, r, x, value).
+template <typename T, int N, int NG, bool TRANS, class Pro, class Epi>
+__global__ void __launch_bounds__(192 * NG) fftw3_inv_pass(const C2<T>* __restrict__ tw3, int nslab, Pro pro, Epi epi,
+                                                          T* tout, Gate gate) {
+    constexpr int E = WaveShape<N>::E, N3 = 3 * N, BPS = N3 / (2 * NG), HT = E / 2;
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, g = wv / 3, wj = wv - 3 * g;
+    C2<T>* grp = reinterpret_cast<C2<T>*>(lds_raw) + g * 3 * wave_region<N>();
+    const int s = blockIdx.x / BPS, r0 = (blockIdx.x - s * BPS) * 2 * NG, ra = r0 + 2 * g;
+    if (s >= nslab || gate.off(s)) return;
+
+    // the pair at k = wj N/2 + lane + 64 t (and N3/2 on wave 2, lane 0), every load issued first
+    T ca[HT + 1], cb[HT + 1], sa[HT + 1], sb[HT + 1];
+#pragma unroll
+    for (int t = 0; t <= HT; t++) {
+        const int k = wj * (N / 2) + lane + 64 * t;
+        if (t < HT || (wj == 2 && lane == 0)) {
+            const bool has_s = k > 0 && 2 * k < N3;
+            ca[t] = pro(s, ra, k);
+            cb[t] = pro(s, ra + 1, k);
+            sa[t] = has_s ? pro(s, ra, N3 - k) : T(0);
+            sb[t] = has_s ? pro(s, ra + 1, N3 - k) : T(0);
+        }
+    }
+#pragma unroll
+    for (int t = 0; t <= HT; t++) {
+        const int k = wj * (N / 2) + lane + 64 * t;
+        if (t < HT || (wj == 2 && lane == 0)) {
+            grp[g3pos<N>(k)] = C2<T>{ca[t] - sb[t], sa[t] + cb[t]};
+            if (k > 0 && 2 * k < N3) grp[g3pos<N>(N3 - k)] = C2<T>{ca[t] + sb[t], cb[t] - sa[t]};
+        }
+    }
+    __syncthreads();
+    C2<T> v[E];
+#pragma unroll
+    for (int r = 0; r < E; r++) v[r] = grp[g3pos<N>(3 * (lane + 64 * r) + wj)];
+    __syncthreads();  // every wave has its subsequence before the regions are reused
+    wave3_dit<N, true>(v, grp, wj, tw3, lane);
+
+    const T sc = T(1) / sqrt(T(N3));
+    if constexpr (!TRANS) {
+#pragma unroll
+        for (int r = 0; r < E; r++) {
+            const int x = wj * N + lane + 64 * r;
+            epi(s, ra, x, v[r].x * sc);
+            epi(s, ra + 1, x, v[r].y * sc);
+        }
+    } else {
+        static_assert(!TRANS || NG > 1, "a transposed store stages 2 NG rows through the block's LDS");
+        constexpr int RB = 2 * NG, RS = stage_stride<NG>();
+        T* stg = reinterpret_cast<T*>(lds_raw);  // [x][row], row stride RS
+#pragma unroll
+        for (int r = 0; r < E; r++) {
+            const int x = wj * N + lane + 64 * r;
+            stg[x * RS + 2 * g] = v[r].x * sc;
+            stg[x * RS + 2 * g + 1] = v[r].y * sc;
+        }
+        __syncthreads();
+        T* o = tout + (size_t)s * N3 * N3 + r0;
+#pragma unroll 4
+        for (int e = threadIdx.x; e < RB * N3; e += 192 * NG) {
+            const int x = e / RB, rr = e - x * RB;
+            o[(size_t)x * N3 + rr] = stg[x * RS + rr];
+        }
+    }
+}
 
 }  // namespace sw

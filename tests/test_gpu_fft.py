@@ -69,7 +69,7 @@ def test_ptospc_spctop_golden(sw):
     close(rvar, gold("testdata/spctop/spctop_ng32_rvar2.bin"), 1e-14)
 
 
-@pytest.mark.parametrize("ng", [18, 24, 30, 48, 64, 96, 128, 256, 512])
+@pytest.mark.parametrize("ng", [18, 24, 30, 48, 64, 96, 128, 256, 512, 1536])
 def test_2d_against_oracle(sw, oracle, ng):
     rng = np.random.default_rng(ng)
     phys = rng.standard_normal(ng * ng)
