@@ -1191,6 +1191,7 @@ struct Engine {
     void coeffs(Fields f, T* sigx, T* sigy, T* cpt1, T* cpt2);
     void cpsource(Fields f, T* sp0);
     T* psolve_source_wave(const T* ps, const T* sp0, const T* sigx, const T* sigy, const T* cpt1, const T* cpt2);
+    T* psolve_source_wave3(const T* ps, const T* sp0, const T* sigx, const T* sigy, const T* cpt1, const T* cpt2);
     // the fused iteration of the wave path with a compile-time nz: derivative rows in the tiled layout
     // (tile_pos, TILE_RB = the rows of one source-fold block), written by the column-solve launch and
     // read by the source fold.  (64-byte runs in FP32 as well, 16-row tiles, made the writer faster and

@@ -1001,4 +1001,139 @@ __global__ void __launch_bounds__(192 * NG) fftw3_inv_pass(const C2<T>* __restri
     }
 }
 
+// fftw_inv_comb_fwd for length N3: for slab group s the block's groups own pairs of rows r; the K =
+// comb.count(s) Hermitian inputs pro(s, k, r, f) (input k the derivative along the transform axis of input
+// k - 1 when comb.deriv_of_prev(s, k)) are transformed inverse, each folded at once into a = a +
+// comb.coef(s, k, r, j) val from a = comb.init(s, r, j), and the accumulated rows transformed forward and
+// stored transposed: element (r, f) at tout[r + N3 f + N3 N3 s].
+template <typename T, int N, int NG, class Pro, class Comb>
+__global__ void __launch_bounds__(192 * NG) fftw3_inv_comb_fwd(const C2<T>* __restrict__ tw3, int nslab, Pro pro, Comb comb,
+                                                              T* tout, Gate gate) {
+    constexpr int E = WaveShape<N>::E, N3 = 3 * N, BPS = N3 / (2 * NG), HT = E / 2;
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, g = wv / 3, wj = wv - 3 * g;
+    C2<T>* grp = reinterpret_cast<C2<T>*>(lds_raw) + g * 3 * wave_region<N>();
+    const int s = blockIdx.x / BPS, r0 = (blockIdx.x - s * BPS) * 2 * NG, ra = r0 + 2 * g;
+    if (s >= nslab || gate.off(s)) return;
+
+    C2<T> acc[E];  // rows ra (.x) and ra + 1 (.y) at j = wj N + lane + 64 r
+#pragma unroll
+    for (int r = 0; r < E; r++) {
+        const int j = wj * N + lane + 64 * r;
+        acc[r] = C2<T>{comb.init(s, ra, j), comb.init(s, ra + 1, j)};
+    }
+    const int K = comb.count(s);
+    const T sc = T(1) / sqrt(T(N3));
+    // the current input's Hermitian pair at k = wj N/2 + lane + 64 t (and N3/2 on wave 2, lane 0)
+    T hca[HT + 1], hcb[HT + 1], hsa[HT + 1], hsb[HT + 1];
+    C2<T> v[E];
+    // the lane index passes through an empty asm in the loop and after it, so that the compiler does not
+    // hoist every lane-dependent address of the loop out of it and hold them in registers
+#pragma unroll 1
+    for (int k = 0; k < K; k++) {
+        int lane = threadIdx.x & 63;
+        asm volatile("" : "+v"(lane));
+        const bool dcur = k > 0 && comb.deriv_of_prev(s, k);
+        if (!dcur) {
+#pragma unroll
+            for (int t = 0; t <= HT; t++) {
+                const int kk = wj * (N / 2) + lane + 64 * t;
+                if (t < HT || (wj == 2 && lane == 0)) {
+                    const bool has_s = kk > 0 && 2 * kk < N3;
+                    hca[t] = pro(s, k, ra, kk);
+                    hcb[t] = pro(s, k, ra + 1, kk);
+                    hsa[t] = has_s ? pro(s, k, ra, N3 - kk) : T(0);
+                    hsb[t] = has_s ? pro(s, k, ra + 1, N3 - kk) : T(0);
+                }
+            }
+        }
+        // pack (the derivative along the transform axis when dcur: -k times the sine part at k, k
+        // times the cosine part at N3 - k)
+#pragma unroll
+        for (int t = 0; t <= HT; t++) {
+            const int kk = wj * (N / 2) + lane + 64 * t;
+            if (t < HT || (wj == 2 && lane == 0)) {
+                T ca = hca[t], cb = hcb[t], sa = hsa[t], sb = hsb[t];
+                if (dcur) {
+                    const T m = (kk == 0 || 2 * kk == N3) ? T(0) : T(kk);
+                    ca = -m * hsa[t];
+                    cb = -m * hsb[t];
+                    sa = m * hca[t];
+                    sb = m * hcb[t];
+                }
+                grp[g3pos<N>(kk)] = C2<T>{ca - sb, sa + cb};
+                if (kk > 0 && 2 * kk < N3) grp[g3pos<N>(N3 - kk)] = C2<T>{ca + sb, cb - sa};
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < E; r++) v[r] = grp[g3pos<N>(3 * (lane + 64 * r) + wj)];
+        __syncthreads();
+        wave3_dit<N, true>(v, grp, wj, tw3, lane);
+        T cx[E], cy[E];
+#pragma unroll
+        for (int r = 0; r < E; r++) {
+            const int j = wj * N + lane + 64 * r;
+            cx[r] = comb.coef(s, k, ra, j);
+            cy[r] = comb.coef(s, k, ra + 1, j);
+        }
+#pragma unroll
+        for (int r = 0; r < E; r++) {
+            acc[r].x = acc[r].x + cx[r] * (v[r].x * sc);
+            acc[r].y = acc[r].y + cy[r] * (v[r].y * sc);
+        }
+    }
+    // forward transform of the accumulated rows: decimate them through the group's regions
+    int lp = threadIdx.x & 63;
+    asm volatile("" : "+v"(lp));
+    const int lane_out = lp;
+#pragma unroll
+    for (int r = 0; r < E; r++) grp[wj * wave_region<N>() + wpad(lane_out + 64 * r)] = acc[r];
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < E; r++) v[r] = grp[g3pos<N>(3 * (lane_out + 64 * r) + wj)];
+    __syncthreads();
+    wave3_dit<N, false>(v, grp, wj, tw3, lane_out);
+#pragma unroll
+    for (int r = 0; r < E; r++) grp[wj * wave_region<N>() + wpad(lane_out + 64 * r)] = v[r];
+    __syncthreads();
+    const T sh = T(0.5) * sc;
+    T ar[HT + 1], ai[HT + 1], br[HT + 1], bi[HT + 1];
+#pragma unroll
+    for (int t = 0; t <= HT; t++) {
+        const int kk = wj * (N / 2) + lane_out + 64 * t;
+        if (t < HT || (wj == 2 && lane_out == 0)) {
+            const int kc = kk == 0 ? 0 : N3 - kk;
+            const C2<T> z = grp[g3pos<N>(kk)], w = grp[g3pos<N>(kc)];
+            ar[t] = (z.x + w.x) * sh;
+            ai[t] = (z.y - w.y) * sh;
+            br[t] = (z.y + w.y) * sh;
+            bi[t] = (w.x - z.x) * sh;
+        }
+    }
+    static_assert(NG > 1, "a transposed store stages 2 NG rows through the block's LDS");
+    constexpr int RB = 2 * NG, RS = stage_stride<NG>();
+    T* stg = reinterpret_cast<T*>(lds_raw);  // [f][row], row stride RS
+    __syncthreads();                         // every group is done with its regions
+#pragma unroll
+    for (int t = 0; t <= HT; t++) {
+        const int kk = wj * (N / 2) + lane_out + 64 * t;
+        if (t < HT || (wj == 2 && lane_out == 0)) {
+            stg[kk * RS + 2 * g] = ar[t];
+            stg[kk * RS + 2 * g + 1] = br[t];
+            if (kk > 0 && 2 * kk < N3) {
+                stg[(N3 - kk) * RS + 2 * g] = ai[t];
+                stg[(N3 - kk) * RS + 2 * g + 1] = bi[t];
+            }
+        }
+    }
+    __syncthreads();
+    T* o = tout + (size_t)s * N3 * N3 + r0;
+#pragma unroll 4
+    for (int e = threadIdx.x; e < RB * N3; e += 192 * NG) {
+        const int f = e / RB, rr = e - f * RB;
+        o[(size_t)f * N3 + rr] = stg[f * RS + rr];
+    }
+}
+
 }  // namespace sw
