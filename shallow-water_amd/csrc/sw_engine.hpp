@@ -1062,6 +1062,16 @@ struct Engine {
     // ng has no wave kernels: the caller then forms them with a 2-D transform pair.
     template <int K>
     bool deriv_lines(int nslab, const DerivIo<T, K>& io) {
+        if (wave3_n()) {
+            if (nslab <= 0) return true;
+            constexpr int N = 512, N3 = 1536;
+            const size_t lds = wave3_lds_bytes<N, NG3, T>();
+            lds_optin((const void*)fftw3_deriv_lines<T, N, NG3, DerivIo<T, K>>, lds);
+            hipLaunchKernelGGL((fftw3_deriv_lines<T, N, NG3, DerivIo<T, K>>), dim3(nslab * (N3 / (2 * NG3))), dim3(192 * NG3), lds,
+                               st, plan.tw, nslab, io);
+            SW_HIP(hipGetLastError());
+            return true;
+        }
         if (!wave_n()) return false;
         if (nslab <= 0) return true;
         wave_dispatch([&](auto nn_) {

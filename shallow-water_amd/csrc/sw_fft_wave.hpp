@@ -1136,4 +1136,102 @@ __global__ void __launch_bounds__(192 * NG) fftw3_inv_comb_fwd(const C2<T>* __re
     }
 }
 
+// fftw_deriv_lines for length N3: the derivative along one axis of real N3 x N3 slabs (physical layout
+// y + N3 x) by a 1-D transform pair along that axis, one group per pair of lines; io.along_x(s) picks
+// the axis (columns y through the block's LDS stage, runs of 2 NG values; rows x straight).
+template <typename T, int N, int NG, class Io>
+__global__ void __launch_bounds__(192 * NG) fftw3_deriv_lines(const C2<T>* __restrict__ tw3, int nslab, Io io) {
+    constexpr int E = WaveShape<N>::E, N3 = 3 * N, BPS = N3 / (2 * NG), RB = 2 * NG, RS = stage_stride<NG>(), HT = E / 2;
+    static_assert(NG > 1, "columns are staged through the block's LDS");
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, g = wv / 3, wj = wv - 3 * g;
+    C2<T>* grp = reinterpret_cast<C2<T>*>(lds_raw) + g * 3 * wave_region<N>();
+    T* stg = reinterpret_cast<T*>(lds_raw);  // column mode: [x][column], row stride RS
+    const int s = blockIdx.x / BPS, r0 = (blockIdx.x - s * BPS) * RB, ra = r0 + 2 * g;
+    if (s >= nslab) return;
+    const bool ax = io.along_x(s);
+    const T* __restrict__ in = io.in(s);
+    T* __restrict__ out = io.out(s);
+
+    C2<T> v[E];
+    if (ax) {
+#pragma unroll 4
+        for (int e = threadIdx.x; e < RB * N3; e += 192 * NG) {
+            const int x = e / RB, rr = e - x * RB;
+            stg[x * RS + rr] = in[(size_t)x * N3 + r0 + rr];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < E; r++) {
+            const int x = 3 * (lane + 64 * r) + wj;
+            v[r] = C2<T>{stg[x * RS + 2 * g], stg[x * RS + 2 * g + 1]};
+        }
+        __syncthreads();  // the stage becomes the groups' regions
+    } else {
+#pragma unroll
+        for (int r = 0; r < E; r++) {
+            const int j = 3 * (lane + 64 * r) + wj;
+            v[r] = C2<T>{in[(size_t)ra * N3 + j], in[(size_t)(ra + 1) * N3 + j]};
+        }
+    }
+    wave3_dit<N, false>(v, grp, wj, tw3, lane);
+#pragma unroll
+    for (int r = 0; r < E; r++) grp[wj * wave_region<N>() + wpad(lane + 64 * r)] = v[r];
+    __syncthreads();
+    // split at k = wj N/2 + lane + 64 t (and N3/2 on wave 2, lane 0), then pack the derivative along the
+    // axis: -k times the sine part at k, k times the cosine part at N3 - k
+    const T sh = T(0.5) / sqrt(T(N3));
+    T ca[HT + 1], sa[HT + 1], cb[HT + 1], sb[HT + 1];
+#pragma unroll
+    for (int t = 0; t <= HT; t++) {
+        const int kk = wj * (N / 2) + lane + 64 * t;
+        if (t < HT || (wj == 2 && lane == 0)) {
+            const int kc = kk == 0 ? 0 : N3 - kk;
+            const C2<T> z = grp[g3pos<N>(kk)], w = grp[g3pos<N>(kc)];
+            ca[t] = (z.x + w.x) * sh;
+            sa[t] = (z.y - w.y) * sh;
+            cb[t] = (z.y + w.y) * sh;
+            sb[t] = (w.x - z.x) * sh;
+        }
+    }
+    __syncthreads();  // every wave has read the spectrum
+#pragma unroll
+    for (int t = 0; t <= HT; t++) {
+        const int kk = wj * (N / 2) + lane + 64 * t;
+        if (t < HT || (wj == 2 && lane == 0)) {
+            const T m = (kk == 0 || 2 * kk == N3) ? T(0) : T(kk);
+            const T dca = -m * sa[t], dcb = -m * sb[t], dsa = m * ca[t], dsb = m * cb[t];
+            grp[g3pos<N>(kk)] = C2<T>{dca - dsb, dsa + dcb};
+            if (kk > 0 && 2 * kk < N3) grp[g3pos<N>(N3 - kk)] = C2<T>{dca + dsb, dcb - dsa};
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < E; r++) v[r] = grp[g3pos<N>(3 * (lane + 64 * r) + wj)];
+    __syncthreads();
+    wave3_dit<N, true>(v, grp, wj, tw3, lane);
+    const T sc = T(1) / sqrt(T(N3));
+    if (ax) {
+#pragma unroll
+        for (int r = 0; r < E; r++) {
+            const int x = wj * N + lane + 64 * r;
+            stg[x * RS + 2 * g] = v[r].x * sc;
+            stg[x * RS + 2 * g + 1] = v[r].y * sc;
+        }
+        __syncthreads();
+#pragma unroll 4
+        for (int e = threadIdx.x; e < RB * N3; e += 192 * NG) {
+            const int x = e / RB, rr = e - x * RB;
+            out[(size_t)x * N3 + r0 + rr] = stg[x * RS + rr];
+        }
+    } else {
+#pragma unroll
+        for (int r = 0; r < E; r++) {
+            const int j = wj * N + lane + 64 * r;
+            out[(size_t)ra * N3 + j] = v[r].x * sc;
+            out[(size_t)(ra + 1) * N3 + j] = v[r].y * sc;
+        }
+    }
+}
+
 }  // namespace sw
