@@ -1034,6 +1034,29 @@ struct Engine {
     template <int NW1 = 0, class Pro, class Mul, class Epi>
     void fwd_mul_inv(int nslab, Pro pro, Mul mul, Epi epi, T* spec = nullptr, int kcut = -1) {
         if (nslab <= 0) return;
+        if (wave3_n()) {
+            // pass 1 along y (transposing), then forward along x, multiply and inverse along kx in one
+            // launch, then the straight pass along ky
+            constexpr int N = 512, N3 = 1536;
+            T* t1 = tmp_for(nslab);
+            T* t2 = W("fmi", (size_t)nslab * n2);
+            const size_t l1 = wave3_lds_bytes<N, NG3, T>(), l2 = wave3_lds_bytes<N, 1, T>();
+            lds_optin((const void*)fftw3_fwd_pass<T, N, NG3, true, Pro, NoEpi>, l1);
+            lds_optin((const void*)fftw3_fwd_mul_inv<T, N, NG3, LoadPlain<T>, Mul>, l1);
+            lds_optin((const void*)fftw3_inv_pass<T, N, 1, false, LoadPlain<T>, Epi>, l2);
+            hipLaunchKernelGGL((fftw3_fwd_pass<T, N, NG3, true, Pro, NoEpi>), dim3(nslab * (N3 / (2 * NG3))), dim3(192 * NG3), l1,
+                               st, plan.tw, nslab, pro, NoEpi{}, t1, Gate{});
+            SW_HIP(hipGetLastError());
+            LoadPlain<T> lp1{t1, n2, ng};
+            hipLaunchKernelGGL((fftw3_fwd_mul_inv<T, N, NG3, LoadPlain<T>, Mul>), dim3(nslab * (N3 / (2 * NG3))), dim3(192 * NG3),
+                               l1, st, plan.tw, nslab, lp1, mul, spec, t2);
+            SW_HIP(hipGetLastError());
+            LoadPlain<T> lp2{t2, n2, ng};
+            hipLaunchKernelGGL((fftw3_inv_pass<T, N, 1, false, LoadPlain<T>, Epi>), dim3(nslab * (N3 / 2)), dim3(192), l2, st,
+                               plan.tw, nslab, lp2, epi, (T*)nullptr, Gate{});
+            SW_HIP(hipGetLastError());
+            return;
+        }
         if (wave_n()) {
             T* t1 = tmp_for(nslab);
             T* t2 = W("fmi", (size_t)nslab * n2);

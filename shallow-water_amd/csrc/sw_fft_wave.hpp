@@ -1234,4 +1234,83 @@ __global__ void __launch_bounds__(192 * NG) fftw3_deriv_lines(const C2<T>* __res
     }
 }
 
+// fftw_fwd_mul_inv for length N3: rows r of slab s (pro(s, r, j)) transformed forward, each coefficient at
+// packed position f multiplied by mul(s, r, f) (and stored at spec[s N3 N3 + r N3 + f] when spec is not
+// null), transformed back and stored transposed: element (r, j) at out[r + N3 j + N3 N3 s].
+template <typename T, int N, int NG, class Pro, class Mul>
+__global__ void __launch_bounds__(192 * NG) fftw3_fwd_mul_inv(const C2<T>* __restrict__ tw3, int nslab, Pro pro, Mul mul,
+                                                             T* spec, T* tout) {
+    constexpr int E = WaveShape<N>::E, N3 = 3 * N, BPS = N3 / (2 * NG), RB = 2 * NG, RS = stage_stride<NG>(), HT = E / 2;
+    static_assert(NG > 1, "a transposed store stages 2 NG rows through the block's LDS");
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, g = wv / 3, wj = wv - 3 * g;
+    C2<T>* grp = reinterpret_cast<C2<T>*>(lds_raw) + g * 3 * wave_region<N>();
+    const int s = blockIdx.x / BPS, r0 = (blockIdx.x - s * BPS) * RB, ra = r0 + 2 * g;
+    if (s >= nslab) return;
+
+    C2<T> v[E];
+#pragma unroll
+    for (int r = 0; r < E; r++) {
+        const int j = 3 * (lane + 64 * r) + wj;
+        v[r] = C2<T>{pro(s, ra, j), pro(s, ra + 1, j)};
+    }
+    wave3_dit<N, false>(v, grp, wj, tw3, lane);
+#pragma unroll
+    for (int r = 0; r < E; r++) grp[wj * wave_region<N>() + wpad(lane + 64 * r)] = v[r];
+    __syncthreads();
+    const T sh = T(0.5) / sqrt(T(N3));
+    T ca[HT + 1], sa[HT + 1], cb[HT + 1], sb[HT + 1];
+#pragma unroll
+    for (int t = 0; t <= HT; t++) {
+        const int kk = wj * (N / 2) + lane + 64 * t;
+        if (t < HT || (wj == 2 && lane == 0)) {
+            const int kc = kk == 0 ? 0 : N3 - kk;
+            const bool has_s = kk > 0 && 2 * kk < N3;
+            const C2<T> z = grp[g3pos<N>(kk)], w = grp[g3pos<N>(kc)];
+            ca[t] = (z.x + w.x) * sh * mul(s, ra, kk);
+            cb[t] = (z.y + w.y) * sh * mul(s, ra + 1, kk);
+            sa[t] = has_s ? (z.y - w.y) * sh * mul(s, ra, N3 - kk) : T(0);
+            sb[t] = has_s ? (w.x - z.x) * sh * mul(s, ra + 1, N3 - kk) : T(0);
+            if (spec) {
+                T* o = spec + (size_t)s * N3 * N3 + (size_t)ra * N3;
+                o[kk] = ca[t];
+                o[N3 + kk] = cb[t];
+                if (has_s) {
+                    o[N3 - kk] = sa[t];
+                    o[2 * N3 - kk] = sb[t];
+                }
+            }
+        }
+    }
+    __syncthreads();  // every wave has read the spectrum
+#pragma unroll
+    for (int t = 0; t <= HT; t++) {
+        const int kk = wj * (N / 2) + lane + 64 * t;
+        if (t < HT || (wj == 2 && lane == 0)) {
+            grp[g3pos<N>(kk)] = C2<T>{ca[t] - sb[t], sa[t] + cb[t]};
+            if (kk > 0 && 2 * kk < N3) grp[g3pos<N>(N3 - kk)] = C2<T>{ca[t] + sb[t], cb[t] - sa[t]};
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < E; r++) v[r] = grp[g3pos<N>(3 * (lane + 64 * r) + wj)];
+    __syncthreads();
+    wave3_dit<N, true>(v, grp, wj, tw3, lane);
+    const T sc = T(1) / sqrt(T(N3));
+    T* stg = reinterpret_cast<T*>(lds_raw);  // [x][row], row stride RS
+#pragma unroll
+    for (int r = 0; r < E; r++) {
+        const int x = wj * N + lane + 64 * r;
+        stg[x * RS + 2 * g] = v[r].x * sc;
+        stg[x * RS + 2 * g + 1] = v[r].y * sc;
+    }
+    __syncthreads();
+    T* o = tout + (size_t)s * N3 * N3 + r0;
+#pragma unroll 4
+    for (int e = threadIdx.x; e < RB * N3; e += 192 * NG) {
+        const int x = e / RB, rr = e - x * RB;
+        o[(size_t)x * N3 + rr] = stg[x * RS + rr];
+    }
+}
+
 }  // namespace sw
