@@ -130,6 +130,13 @@ int sw_savegrid(sw_state* st, float* d3, float* d2, double* spec3);
  * formats.  *iters_total = the psolve iterations of the run. */
 int sw_nhswps(const char* dir, size_t ng, size_t nz, double time_step, double duration, double save_interval,
               int precision, int* iters_total);
+/* balinit (src/balinit.rs:21-436): the balanced initial state of a 2-D PV anomaly.  qq_init: ng*ng values
+   (the record of qq_init.r8); out: the balanced q, delta and gamma (the three records of sw_init.r8),
+   physical, ng*ng each; iters: the iterations run (may be null).  Any precision handle: it runs in FP64. */
+int sw_balinit(sw_spectral* s, const double* qq_init, double* qq, double* dd, double* gg, int* iters);
+/* balinit(&Parameters) as the program runs it: reads dir/qq_init.r8, writes dir/sw_init.r8 (balinit.rs:28-36,
+   419-433) */
+int sw_balinit_file(const char* dir, size_t ng, size_t nz, int* iters);
 /* FP64 checkpoint of a state in the r8 layout: an 8-byte header holding t, then the 16 fields in the order
  * of the State (no reference counterpart: the reference keeps only the t = 0 *_init.r8 files). */
 int sw_state_save(sw_state* st, const char* path);

@@ -53,6 +53,9 @@ int orc_energy(void* s, double** fields, double t, double* out6);
 int orc_savegrid(void* s, double** fields, float* d3, float* d2, double* spec3);
 int orc_run(void* s, double** fields, double* t, double tsim, double tgsave, int nitmax, double* mon, int maxmon,
             int* nmon, double* eco, int maxeco, int* neco, int* iters_total);
+/* balinit (src/balinit.rs:21-436): the balanced q, delta, gamma (the three records of sw_init.r8) of the
+ * 2-D PV anomaly qq_init (the record of qq_init.r8), ng*ng each; returns the iteration count (>= 1) */
+int orc_balinit(void* s, const double* qq_init, double* qq, double* dd, double* gg);
 #ifdef __cplusplus
 }
 #endif

@@ -162,6 +162,9 @@ int orc_spec1d(void* s, const double* ss, double* spec) {
     static_cast<Spectral*>(s)->spec1d(ss, spec);
     return 0;
 }
+int orc_balinit(void* s, const double* qq_init, double* qq, double* dd, double* gg) {
+    return balinit(*static_cast<Spectral*>(s), qq_init, qq, dd, gg);
+}
 int orc_vertical(void* s, double** fields) {
     State st = make_state(s, fields, 0.0, 0);
     vertical(st);

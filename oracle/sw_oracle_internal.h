@@ -81,4 +81,7 @@ int run(State& st, double tsim, double tgsave, int nitmax, double* mon, int maxm
 
 double unrolled_sum(const double* x, size_t n);
 
+// src/balinit.rs:21-436 (2-D; returns the iteration count)
+int balinit(const Spectral& S, const double* qq_init, double* qq_out, double* dd_out, double* gg_out);
+
 }  // namespace orc

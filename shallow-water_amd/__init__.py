@@ -4,7 +4,7 @@ A thin ctypes layer over libsw_hip.so (C ABI in include/shallow_water.h) that mi
 crate's API for this path: stafft (factorisen, initfft, forfft, revfft), sta2dfft (D2FFT with
 ptospc, spctop, xderiv, yderiv), spectral (Spectral with main_invert, jacob, divs, ptospc3d,
 spctop3d, deal3d, deal2d) and nhswps (State with vertical, coeffs, cpsource, psolve, source,
-diagnose, advance).  Arrays are float64 numpy arrays in the reference layout (element (i,j,iz) at
+diagnose, advance) and balinit.  Arrays are float64 numpy arrays in the reference layout (element (i,j,iz) at
 i + ng*j + ng*ng*iz).  Errors raise SwError carrying the library's sw_status code where the
 reference would exit(1) or panic.
 
@@ -353,6 +353,26 @@ def nhswps(out_dir, ng, nz, time_step, duration, save_interval, precision=FP64):
     it = C.c_int()
     _check(_lib.sw_nhswps(os.fsencode(out_dir), C.c_size_t(ng), C.c_size_t(nz), C.c_double(time_step),
                           C.c_double(duration), C.c_double(save_interval), C.c_int(precision), C.byref(it)))
+    return it.value
+
+
+def balinit(spectral, qq_init):
+    """balinit (src/balinit.rs:21-436) on the GPU for one ng*ng PV anomaly: the balanced (qq, dd, gg)
+    that sw_init.r8 holds, and the iteration count."""
+    q0 = np.ascontiguousarray(qq_init, np.float64).ravel()
+    if q0.size != spectral.ng * spectral.ng:
+        raise ValueError("qq_init must hold ng*ng values")
+    qq, dd, gg = spectral.slab(), spectral.slab(), spectral.slab()
+    it = C.c_int()
+    _check(_lib.sw_balinit(spectral._h, _d(q0), _d(qq), _d(dd), _d(gg), C.byref(it)))
+    return qq, dd, gg, it.value
+
+
+def balinit_file(out_dir, ng, nz):
+    """balinit(&Parameters) as the program runs it: reads out_dir/qq_init.r8 and writes out_dir/sw_init.r8.
+    Returns the iteration count."""
+    it = C.c_int()
+    _check(_lib.sw_balinit_file(os.fsencode(out_dir), C.c_size_t(ng), C.c_size_t(nz), C.byref(it)))
     return it.value
 
 

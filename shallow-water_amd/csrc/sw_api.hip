@@ -897,6 +897,41 @@ int sw_state_load(sw_state* st, const char* path) {
     });
 }
 
+int sw_balinit(sw_spectral* s, const double* qq_init, double* qq, double* dd, double* gg, int* iters) {
+    return guard([&] {
+        return with_engine(s, [&](auto& e) {
+            Arg<double> a(qq_init, e.n2, true, false, e.st), q(qq, e.n2, false, true, e.st), d(dd, e.n2, false, true, e.st),
+                g(gg, e.n2, false, true, e.st);
+            const int it = e.balinit(a.d, q.d, d.d, g.d);
+            q.finish();
+            d.finish();
+            g.finish();
+            if (iters) *iters = it;
+            return 0;
+        });
+    });
+}
+
+int sw_balinit_file(const char* dir, size_t ng, size_t nz, int* iters) {
+    return guard([&] {
+        SW_CHECK(dir != nullptr, SW_ERR_ARG, "null directory");
+        const std::string d(dir);
+        const size_t n2 = ng * ng;
+        std::vector<double> q0(n2), out(3 * (n2 + 1), 0.0);
+        read_r8(d + "/qq_init.r8", q0.data(), n2);
+        sw_spectral* sp = nullptr;
+        int rc = sw_spectral_create(ng, nz, SW_FP64, &sp);
+        if (rc) throw Error(rc, g_err);
+        std::unique_ptr<sw_spectral, int (*)(sw_spectral*)> sp_guard(sp, sw_spectral_destroy);
+        // sw_init.r8 (balinit.rs:419-433): three records, each a zero header and ng^2 values: q, delta, gamma
+        rc = sw_balinit(sp, q0.data(), out.data() + 1, out.data() + n2 + 2, out.data() + 2 * n2 + 3, iters);
+        if (rc) throw Error(rc, g_err);
+        File f(d + "/sw_init.r8", "wb");
+        f.write(out.data(), out.size() * sizeof(double));
+        return 0;
+    });
+}
+
 int sw_nhswps(const char* dir, size_t ng, size_t nz, double time_step, double duration, double save_interval, int precision,
               int* iters_total) {
     return guard([&] {

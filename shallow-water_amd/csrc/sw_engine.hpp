@@ -1255,6 +1255,9 @@ struct Engine {
     DevBuf<int> shell_idx, shell_off;
     DevBuf<double> shell_w;
     void energy(Fields f, double t, double out[6]);
+    // balinit (src/balinit.rs:21-436) on one ng x ng slab: qq0 in, the balanced qq, dd, gg out (physical,
+    // device arrays); returns the iteration count
+    int balinit(const T* qq0, T* qq, T* dd, T* gg);
     // observer of the time loop: a monitor row per step, and each save (after its energies, with the
     // state's fields inverted and the pressure solved)
     struct RunHooks {

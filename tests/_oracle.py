@@ -24,7 +24,7 @@ def lib():
         for name in ("orc_spectral_free", "orc_spectral_get", "orc_main_invert", "orc_jacob", "orc_divs",
                      "orc_ptospc3d", "orc_spctop3d", "orc_deal3d", "orc_deal2d", "orc_spec1d",
                      "orc_vertical", "orc_coeffs", "orc_cpsource", "orc_psolve", "orc_source",
-                     "orc_diagnose", "orc_advance", "orc_energy", "orc_run", "orc_savegrid"):
+                     "orc_diagnose", "orc_advance", "orc_energy", "orc_run", "orc_savegrid", "orc_balinit"):
             getattr(_lib, name).argtypes = None
     return _lib
 
@@ -146,6 +146,14 @@ class Spectral:
 
     def spec1d(self, ss, spec):
         lib().orc_spec1d(self._h, dp(ss), dp(spec))
+
+    def balinit(self, qq_init):
+        """src/balinit.rs:21-436 on one ng*ng PV anomaly: (qq, dd, gg, iterations)"""
+        q0 = np.ascontiguousarray(qq_init, np.float64).ravel()
+        assert q0.size == self.ng * self.ng
+        qq, dd, gg = self.slab(), self.slab(), self.slab()
+        it = lib().orc_balinit(self._h, dp(q0), dp(qq), dp(dd), dp(gg))
+        return qq, dd, gg, it
 
 
 class State:

@@ -4,7 +4,8 @@ The fixtures are data files held by the reference's own tests (big-endian f64, F
 no header; shapes encoded in the file name), plus the end-to-end case of src/test.rs:27-36 for
 ng = 32, nz = 4: its balanced initial state (testdata/balinit/32_4_sw_init.r8, three little-endian
 f64 records of an 8-byte header and ng^2 values: q, delta, gamma) and the run's outputs
-(testdata/complete/32_4: monitor.asc, ecomp.asc and the .r4 field dumps).  This script copies them verbatim, converts the
+(testdata/complete/32_4: monitor.asc, ecomp.asc and the .r4 field dumps), and the two balinit
+cases of src/balinit.rs:441-481 (qq_init.r8 in, sw_init.r8 out, at 18x2 and 32x4).  This script copies them verbatim, converts the
 insta snapshot values (factor lists and wavenumber arrays) into snapshots.json, and writes
 MANIFEST.json with the SHA-256 and source path of every file.  It is the only script that reads
 the reference tree; nothing at test time does.
@@ -29,7 +30,8 @@ DIRS = [
 ]
 
 # single files of the end-to-end case (src/test.rs:27-36)
-EXTRA = ["testdata/balinit/32_4_sw_init.r8", "testdata/complete/32_4/monitor.asc",
+EXTRA = ["testdata/balinit/32_4_sw_init.r8", "testdata/balinit/32_4_qq_init.r8",
+         "testdata/balinit/18_2_qq_init.r8", "testdata/balinit/18_2_sw_init.r8", "testdata/complete/32_4/monitor.asc",
          "testdata/complete/32_4/ecomp.asc"] + \
         [f"testdata/complete/32_4/2d/{f}.r4" for f in ("d", "g", "h", "q", "zeta")] + \
         [f"testdata/complete/32_4/3d/{f}.r4" for f in ("d", "g", "pn", "ql", "r", "w")]

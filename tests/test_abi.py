@@ -21,7 +21,7 @@ def test_header_declares_the_reference_surface():
     names = declared()
     for n in ("sw_forfft", "sw_revfft", "sw_ptospc", "sw_spctop", "sw_xderiv", "sw_yderiv", "sw_main_invert",
               "sw_deal2d", "sw_deal3d", "sw_advance", "sw_source", "sw_psolve", "sw_coeffs", "sw_vertical",
-              "sw_ensemble_create", "sw_ensemble_advance", "sw_ensemble_destroy"):
+              "sw_ensemble_create", "sw_ensemble_advance", "sw_ensemble_destroy", "sw_balinit", "sw_balinit_file"):
         assert n in names
 
 
