@@ -19,6 +19,8 @@ the oracle (C++ restatement of the reference algorithm, OpenMP over levels, buil
 this host when a compiler is there) for one warm-up and three timed steps of the same workload.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--ng 512] [--nz 16]
+       python bench.py --large 200   (the ng=1536, nz=24 workload alone: mixed and FP64, 200 steps)
+       python bench.py --ensemble-only   (the 64-member ensemble workload alone)
 """
 import argparse
 import json
@@ -146,9 +148,10 @@ def one_step_errors(sw, ng, nz, init):
     return err
 
 
-def seeded_init(sw, ng, nz, seed, zeta_frac):
-    """initial_fields with rms zeta = zeta_frac * f."""
-    qs, ds = sw.synth_kminus3(ng, nz, seed=seed, k0=8.0, zeta_rms=zeta_frac * 4 * np.pi)
+def seeded_init(sw, ng, nz, seed, zeta_frac, delta_frac=0.0):
+    """initial_fields with rms zeta = zeta_frac * f (and an independent random delta of rms delta_frac * f)."""
+    qs, ds = sw.synth_kminus3(ng, nz, seed=seed, k0=8.0, zeta_rms=zeta_frac * 4 * np.pi,
+                              delta_rms=delta_frac * 4 * np.pi)
     gs = np.zeros_like(qs)
     spec = sw.Spectral(ng, nz, sw.FP64)
     r, u, v, zeta = spec.field(), spec.field(), spec.field(), spec.field()
@@ -201,6 +204,57 @@ def long_run_errors(sw, ng, nz, init, steps):
     err["h"] = rel_l2(mix["z"][nz * n2:], f64["z"][nz * n2:])
     return {"steps_requested": steps, "steps_compared": target, "stopped": stopped, "rel_l2_mixed_vs_fp64": err,
             "fp64_monitor_t_zrms2half_zrms_zmax_umax": diag}
+
+
+def large_bench(sw, steps, seed=1, ng=1536, nz=24, zeta_frac=0.5):
+    """The large mixed-radix workload: ng = 1536 (2^9 3), nz = 24, seeded k^-3 PV (k0 = 8, rms zeta =
+    zeta_frac f) plus an independent random delta of rms 0.01 f, `steps` steps in the mixed mode and in FP64
+    from the same state (each step timed; a run stops early where psolve's iteration stops converging, as
+    the reference exits there, src/nhswps/psolve.rs:360-366).  Reports ms per step and point-steps/s of each
+    mode over the steps it completed, psolve iterations, and the mixed-vs-FP64 errors after 1 step and after
+    the last step when both modes completed every step."""
+    init = seeded_init(sw, ng, nz, seed, zeta_frac, 0.01)
+    points = ng * ng * (nz + 1)
+    out = {"workload": f"nhswps advance, ng={ng}, nz={nz}, seeded k^-3 PV (k0=8, rms zeta={zeta_frac} f) + random "
+                       f"delta (rms 0.01 f)", "steps_requested": steps}
+    fields = {}
+    for p, name in ((sw.MIXED, "mixed"), (sw.FP64, "fp64")):
+        spec, st = new_state(sw, ng, nz, p, init)
+        its, times, stop, snap1 = [], [], None, None
+        for k in range(steps):
+            st.set_time(st.t, True)
+            t0 = time.perf_counter()
+            try:
+                _, it = sw.advance(st)
+                sw.sync()
+            except sw.SwError as e:
+                stop = f"step {k + 1}: {e}"
+                break
+            times.append(time.perf_counter() - t0)
+            its.append(sum(it))
+            if k == 0:
+                snap1 = {f: st.get(f) for f in ("qs", "ds", "gs", "z", "pn")}
+        fields[name] = (snap1, {f: st.get(f) for f in ("qs", "ds", "gs", "z", "pn")} if stop is None else None,
+                        len(times))
+        timed = times[1:] if len(times) > 1 else times  # the first step holds one-off set-up
+        ms = 1e3 * sum(timed) / max(len(timed), 1)
+        out[name] = {"steps_completed": len(times), "stopped": stop, "ms_per_step": ms,
+                     "point_steps_per_s": points / (ms * 1e-3) if ms > 0 else None,
+                     "psolve_iters_per_step_first_last": [its[:3], its[-3:]]}
+        print(f"[bench] large {name}: {out[name]}", file=sys.stderr, flush=True)
+        del st, spec
+    n2 = ng * ng
+
+    def errs(a, b):
+        e = {k: rel_l2(a[f], b[f]) for k, f in (("q", "qs"), ("delta", "ds"), ("gamma", "gs"), ("p", "pn"))}
+        e["h"] = rel_l2(a["z"][nz * n2:], b["z"][nz * n2:])
+        return e
+
+    out["rel_l2_mixed_vs_fp64_after_1_step"] = errs(fields["mixed"][0], fields["fp64"][0])
+    if fields["mixed"][1] is not None and fields["fp64"][1] is not None:
+        out[f"rel_l2_mixed_vs_fp64_after_{steps}_steps"] = errs(fields["mixed"][1], fields["fp64"][1])
+    out["mixed_over_fp64"] = out["fp64"]["ms_per_step"] / out["mixed"]["ms_per_step"]
+    return out
 
 
 def ensemble_bench(sw, steps, warmup, nmember=64, runs=((0, "batched"), (1, "batched"), (4, "eager"))):
@@ -324,6 +378,9 @@ def main():
     ap.add_argument("--only", choices=["fp64", "mixed"], default=None,
                     help="only advance the state --steps steps in this precision after --warmup steps, with no "
                          "report (the run that rocprofv3 profiles)")
+    ap.add_argument("--large", type=int, default=0, metavar="STEPS",
+                    help="only the large mixed-radix workload (ng=1536, nz=24, k^-3 PV + random delta): STEPS steps "
+                         "mixed and FP64, errors at step 1 and at the end")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (twice) and exit: the reference reading "
                          "for the FETCH_SIZE/WRITE_SIZE counters")
@@ -345,6 +402,13 @@ def main():
               "run with --gpus 1", file=sys.stderr)
         sys.exit(2)
     sw = load_sw()
+    if args.large:
+        # the bench state (rms zeta = f/2), on which the reference algorithm's pressure iteration stops
+        # converging within the run, then rms zeta = 0.3 f, on which it completes every step
+        print(json.dumps({"large": {"rms_zeta_0.5f": large_bench(sw, args.large, args.seed),
+                                    "rms_zeta_0.3f": large_bench(sw, args.large, args.seed, zeta_frac=0.3)}}),
+              flush=True)
+        return
     if args.ensemble_only:
         ens = ensemble_bench(sw, args.ensemble_steps, 2, runs=((args.ensemble_streams, args.ensemble_mode),))
         print(json.dumps({"ensemble": ens}), flush=True)
