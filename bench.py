@@ -124,9 +124,10 @@ def kernel_names(ng, nz, precision):
     derivative inputs) and the column solve that also forms the next iteration's derivative rows."""
     t = "double" if precision == "fp64" else "float"
     lv, nd = nz + 1, 1 + 3 * nz
-    return {"psolve_deriv": f"fftw_rows_cols_inv<{t}, {ng}, 8, {lv}, {nd}, 8, load-only: first-guess derivative rows>",
-            "psolve_comb": f"fftw_inv_comb_fwd<{t}, {ng}, 4, psolve source fold, tiled inputs>",
-            "psolve_tri": f"fftw_rows_cols_inv<{t}, {ng}, 8, {lv}, {nd}, 8, psolve column solve + derivative rows>"}
+    nw = 4 if precision == "fp64" else 8  # waves per fold block; tile rows 2 nw
+    return {"psolve_deriv": f"fftw_rows_cols_inv<{t}, {ng}, 8, {lv}, {nd}, {2 * nw}, load-only: first-guess derivative rows>",
+            "psolve_comb": f"fftw_inv_comb_fwd<{t}, {ng}, {nw}, psolve source fold, tiled inputs>",
+            "psolve_tri": f"fftw_rows_cols_inv<{t}, {ng}, 8, {lv}, {nd}, {2 * nw}, psolve column solve + derivative rows>"}
 
 
 def rel_l2(a, b):

@@ -1011,8 +1011,9 @@ struct Engine {
     }
     // fftw_inv_comb_fwd over nslab slab groups (transposed output); bytes = the algorithmic bytes of
     // the launch, for the profiling mode that times this site.
-    // 4-wave blocks: in FP64 the kernel runs three waves per SIMD, so three blocks share a CU
-    static constexpr int NWC = 4;
+    // FP64: 4-wave blocks (the kernel runs three waves per SIMD, so three blocks share a CU); FP32:
+    // 8-wave blocks, so that a block's 16 rows are one 16-row tile of the tiled derivative layout below
+    static constexpr int NWC = sizeof(T) == 4 ? 8 : 4;
     template <int NN, class Pro, class Comb>
     void inv_comb_fwd_wave(int nslab, Pro pro, Comb comb, T* out, double bytes, int site) {
         const size_t lds_t = wave_lds_bytes<NN, NWC, T>();
@@ -1226,9 +1227,11 @@ struct Engine {
     T* psolve_source_wave(const T* ps, const T* sp0, const T* sigx, const T* sigy, const T* cpt1, const T* cpt2);
     T* psolve_source_wave3(const T* ps, const T* sp0, const T* sigx, const T* sigy, const T* cpt1, const T* cpt2);
     // the fused iteration of the wave path with a compile-time nz: derivative rows in the tiled layout
-    // (tile_pos, TILE_RB = the rows of one source-fold block), written by the column-solve launch and
-    // read by the source fold.  (64-byte runs in FP32 as well, 16-row tiles, made the writer faster and
-    // the fold's strided two-row loads slower by as much.)
+    // (tile_pos, TILE_RB = the rows of one source-fold block: 8 in FP64, 16 in FP32, so that the
+    // column-solve launch stores runs of 64 bytes in either precision), written by the column-solve launch
+    // and read by the source fold.  In FP32 the 16-row tiles against 8-row ones: the column solve 41.7 ->
+    // 36.0 us and the first-guess rows 39.6 -> 25.7 us per launch, the fold (whose two-row loads are then
+    // 64 bytes apart) 39.6 -> 45.7 us, at 512x16: psolve's launches 3% faster per step.
     static constexpr int TILE_RB = 2 * NWC;
     bool psolve_fused() const { return wave_n() && nz_spec(); }
     void psolve_deriv_tiled(const T* ps, T* D);

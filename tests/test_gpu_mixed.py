@@ -2,7 +2,7 @@
 with the pressure tridiagonal and the residual in FP64; the state and every other stage in FP64)
 against FP64, per field after one step: q (qs), delta (ds), gamma (gs), h (z at the top level) and p
 (pn).  Against the oracle at 128x8 and against the FP64 GPU step at 512x16, from the benchmark's seeded
-k^-3 state.  Every field must be within the 1e-5 bar, and within a regression bound of about twice the
+k^-3 state, and at 1536x24 (three-wave transforms) from the large workload's state.  Every field must be within the 1e-5 bar, and within a regression bound of about twice the
 error measured on MI355X; each test prints its errors."""
 import numpy as np
 import pytest
@@ -54,5 +54,30 @@ def test_mixed_one_step_vs_fp64_512x16(sw, oracle):
     print(f"mixed {ng}x{nz} one step vs FP64 GPU:", {k: f"{v:.2e}" for k, v in err.items()}, flush=True)
     # measured: q 4.6e-11, delta 2.9e-7, gamma 7.3e-7, h 5.2e-12, p 1.0e-7
     bound = {"q": 1e-10, "delta": 6e-7, "gamma": 1.5e-6, "h": 1.2e-11, "p": 2.5e-7}
+    for k, v in err.items():
+        assert v < min(bound[k], 1e-5), (k, v)
+
+
+@pytest.mark.timeout(300)
+def test_mixed_one_step_vs_fp64_1536x24(sw):
+    """The large mixed-radix workload (ng = 1536 on three-wave transforms, nz = 24): seeded k^-3 PV (rms
+    zeta = f/2) plus an independent random delta of rms 0.01 f, inverted by the FP64 GPU main_invert."""
+    ng, nz = 1536, 24
+    qs, ds = sw.synth_kminus3(ng, nz, seed=1, k0=8.0, zeta_rms=0.5 * 4 * np.pi, delta_rms=0.01 * 4 * np.pi)
+    gs = np.zeros_like(qs)
+    s64 = sw.Spectral(ng, nz, sw.FP64)
+    r, u, v, zeta = s64.field(), s64.field(), s64.field(), s64.field()
+    s64.main_invert(qs, ds, gs, r, u, v, zeta)
+    f = dict(qs=qs, ds=ds, gs=gs, r=r, u=u, v=v, zeta=zeta)
+    d = sw.State(s64, ggen=True, **f)
+    sw.advance(d)
+    df = {n: d.get(n) for n in ("qs", "ds", "gs", "z", "pn")}
+    del d
+    m = sw.State(sw.Spectral(ng, nz, sw.MIXED), ggen=True, **f)
+    sw.advance(m)
+    err = field_errors(m.get, lambda n: df[n], ng, nz)
+    print(f"mixed {ng}x{nz} one step vs FP64 GPU:", {k: f"{v:.2e}" for k, v in err.items()}, flush=True)
+    # measured: q 2.1e-10, delta 4.6e-8, gamma 1.1e-7, h 8.7e-13, p 1.4e-7
+    bound = {"q": 5e-10, "delta": 1e-7, "gamma": 2.5e-7, "h": 2e-12, "p": 3e-7}
     for k, v in err.items():
         assert v < min(bound[k], 1e-5), (k, v)
