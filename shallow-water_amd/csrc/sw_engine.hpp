@@ -1223,7 +1223,10 @@ struct Engine {
     void main_invert(const T* qs, const T* ds, const T* gs, T* r, T* u, T* v, T* zeta);
     void vertical(Fields f);
     void coeffs(Fields f, T* sigx, T* sigy, T* cpt1, T* cpt2);
-    void cpsource(Fields f, T* sp0);
+    template <typename U>
+    void coeffs_from(const U* ri, const U* zx, const U* zy, T* sigx, T* sigy, T* cpt1, T* cpt2);
+    // sp0f: when not null, an FP32 copy of sp0 written beside it (the mixed mode's FP32 engine reads it)
+    void cpsource(Fields f, T* sp0, float* sp0f = nullptr);
     T* psolve_source_wave(const T* ps, const T* sp0, const T* sigx, const T* sigy, const T* cpt1, const T* cpt2);
     T* psolve_source_wave3(const T* ps, const T* sp0, const T* sigx, const T* sigy, const T* cpt1, const T* cpt2);
     // the fused iteration of the wave path with a compile-time nz: derivative rows in the tiled layout
