@@ -63,3 +63,16 @@ def test_balinit_matches_oracle(sw, oracle, ng):
     print(f"balinit {ng}: iterations GPU {it} oracle {oit}, rel L2 vs oracle {errs}")
     assert it == oit
     assert all(e < SYNTH_TOL for e in errs.values()), errs
+
+
+def test_balinit_file_errors(sw, tmp_path):
+    """A missing qq_init.r8, or one whose size is not an 8-byte header plus ng^2 values, is SW_ERR_ARG with
+    the reason, and no sw_init.r8 is written (the reference returns the I/O error, balinit.rs:28-36)."""
+    with pytest.raises(sw.SwError) as e:
+        sw.balinit_file(str(tmp_path), 32, 4)
+    assert e.value.code == -1 and "qq_init.r8" in str(e.value)
+    shutil.copyfile(os.path.join(CASE, "18_2_qq_init.r8"), tmp_path / "qq_init.r8")
+    with pytest.raises(sw.SwError) as e:
+        sw.balinit_file(str(tmp_path), 32, 4)
+    assert e.value.code == -1 and "expected" in str(e.value)
+    assert not (tmp_path / "sw_init.r8").exists()

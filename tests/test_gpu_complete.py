@@ -9,6 +9,7 @@ not compare (too noisy for bitwise checks), is compared with the oracle's savegr
 Also: savegrid's output (sw_savegrid) against the oracle's at a wave-kernel size, and the FP64
 checkpoint (sw_state_save / sw_state_load) round trip."""
 import os
+import shutil
 
 import numpy as np
 import pytest
@@ -42,6 +43,26 @@ def run_32_4(sw, tmp_path_factory):
     its = sw.nhswps(out, 32, 4, 1.0 / 32, 1.0, 0.25)
     print(f"nhswps 32x4 to t = 1: {its} psolve iterations", flush=True)
     return out
+
+
+def test_pipeline_from_qq_init_32_4(sw, tmp_path):
+    """The reference's pipeline after vstrip, all on the GPU: balinit from its 32_4_qq_init.r8 (sw_balinit_file
+    writes sw_init.r8), the layer copy of swto3d, then nhswps to t = 1.0: monitor.asc and ecomp.asc token
+    for token as the reference's."""
+    ng, nz = 32, 4
+    out = str(tmp_path)
+    shutil.copyfile(os.path.join(CASE, "balinit", "32_4_qq_init.r8"), os.path.join(out, "qq_init.r8"))
+    assert sw.balinit_file(out, ng, nz) == 9
+    raw = np.fromfile(os.path.join(out, "sw_init.r8"), dtype="<f8").reshape(3, ng * ng + 1)
+    for name, rec in zip(("qq", "dd", "gg"), raw[:, 1:]):
+        with open(os.path.join(out, f"{name}_init.r8"), "wb") as f:
+            f.write(np.zeros(1, "<f8").tobytes() + np.tile(rec, nz + 1).astype("<f8").tobytes())
+    sw.nhswps(out, ng, nz, 1.0 / 32, 1.0, 0.25)
+    for name in ("monitor.asc", "ecomp.asc"):
+        mine = open(os.path.join(out, name)).read().split()
+        ref = open(os.path.join(CASE, "complete", "32_4", name)).read().split()
+        diff = [(i, a, b) for i, (a, b) in enumerate(zip(mine, ref)) if a != b]
+        assert len(mine) == len(ref) and not diff, (name, diff[:5])
 
 
 def test_nhswps_32_4_monitor_and_ecomp(run_32_4):

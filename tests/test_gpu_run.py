@@ -3,7 +3,9 @@
 The loop is the 32x4 strip case: q = A (1 + 0.1 cos x) for |y| < 0.5, zero elsewhere, A = 2 pi, the same
 on every level, delta = gamma = 0, through the loader's path (ptospc3d, zero mean, 2/3 filter:
 src/nhswps/mod.rs:238-275), 20 steps with saves every 5 steps.  (With A = 4 pi the reference
-algorithm's pressure iteration diverges on the first solve: the oracle stops with the same error.)  Monitor and energy rows are compared
+algorithm's pressure iteration diverges on the first solve, balanced by balinit or not: the oracle
+stops with the same error.  The reference's own 4 pi case is its narrower parabolic strip, which the
+end-to-end tests run from its qq_init.r8.)  Monitor and energy rows are compared
 at relative 1e-10; the final fields at the one-step bounds times 20 (rounding differences compound
 over the steps)."""
 import numpy as np
@@ -15,21 +17,32 @@ pytestmark = pytest.mark.gpu
 ALL = ["ri", "r", "u", "v", "w", "zeta", "z", "zx", "zy", "ps", "pn", "dpn", "aa", "qs", "ds", "gs"]
 
 
-def strip_state(sw, ng, nz):
+def strip2d(ng, amp):
+    """q = amp (1 + 0.1 cos x) for |y| < 0.5, zero elsewhere, mean removed; element (i = y, j = x) at i + ng j."""
     x = 2 * np.pi * np.arange(ng) / ng - np.pi
     y = x.copy()
-    qq2 = np.where(np.abs(y)[:, None] < 0.5, 2 * np.pi * (1 + 0.1 * np.cos(x))[None, :], 0.0)  # [i=y, j=x]
+    qq2 = np.where(np.abs(y)[:, None] < 0.5, amp * (1 + 0.1 * np.cos(x))[None, :], 0.0)  # [i=y, j=x]
     qq2 -= qq2.mean()
-    phys = np.tile(qq2.T.ravel(), nz + 1)  # element (i, j) at i + ng j
-    s = sw.Spectral(ng, nz)
-    qs = s.field()
-    s.ptospc3d(phys, qs, 0, nz)
+    return qq2.T.ravel()
+
+
+def load3d(lib, ng, nz, rec):
+    """A 2-D record copied to every level (swto3d, src/swto3d.rs:10-39), then the loader's path: to spectral
+    space, the (0, 0) mode zeroed, times the filter (src/nhswps/mod.rs:238-275); lib: the GPU package or the
+    oracle."""
+    s = lib.Spectral(ng, nz)
+    fs = s.field()
+    s.ptospc3d(np.tile(rec, nz + 1), fs, 0, nz)
     filt = s.get("filt")
     n2 = ng * ng
     for iz in range(nz + 1):
-        qs[iz * n2] = 0.0
-        qs[iz * n2:(iz + 1) * n2] *= filt
-    return qs
+        fs[iz * n2] = 0.0
+        fs[iz * n2:(iz + 1) * n2] *= filt
+    return fs
+
+
+def strip_state(sw, ng, nz):
+    return load3d(sw, ng, nz, strip2d(ng, 2 * np.pi))
 
 
 def test_energy_matches_oracle(sw, oracle):
@@ -55,3 +68,4 @@ def test_run_strip_32x4_20_steps(sw, oracle):
     assert abs(g.t - o.t) < 1e-14
     for f in ALL:
         assert rel_l2(g.get(f), getattr(o, f)) < 20 * STEP_TOL[f], f
+
