@@ -54,6 +54,8 @@ const char* sw_last_error(void);    /* text of the last error on this thread */
 /* ---- 1-D FFT: src/stafft/mod.rs ---- */
 int sw_factorisen(size_t n, size_t factors[5]);                       /* factorisen  mod.rs:97  */
 int sw_initfft(size_t n, size_t factors[5], double* trig);            /* initfft     mod.rs:67  (host) */
+/* forfft/revfft: trig must be initfft(n)'s table (compared value for value to 1e-12, host or device
+   memory; SW_ERR_ARG otherwise); factors may be null */
 int sw_forfft(size_t m, size_t n, double* x, const double* trig,      /* forfft      mod.rs:158 */
               const size_t factors[5]);
 int sw_revfft(size_t m, size_t n, double* x, const double* trig,      /* revfft      mod.rs:301 */

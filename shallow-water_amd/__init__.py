@@ -87,14 +87,20 @@ def initfft(n):
 
 
 def forfft(m, n, x, trig=None, factors=None):
-    """In place: m real transforms of length n of x(m,n) (element (i,j) at i + m*j)."""
+    """In place: m real transforms of length n of x(m,n) (element (i,j) at i + m*j).  trig must be
+    initfft(n)'s table (checked; SwError otherwise); None: that table, formed here."""
+    if trig is None:
+        trig = initfft(n)[1]
     fa, fp = _sz(factors) if factors is not None else (None, None)
-    _check(_lib.sw_forfft(C.c_size_t(m), C.c_size_t(n), _d(x), None if trig is None else _d(trig), fp))
+    _check(_lib.sw_forfft(C.c_size_t(m), C.c_size_t(n), _d(x), _d(trig), fp))
 
 
 def revfft(m, n, x, trig=None, factors=None):
+    """In place: the inverse of forfft (same arguments)."""
+    if trig is None:
+        trig = initfft(n)[1]
     fa, fp = _sz(factors) if factors is not None else (None, None)
-    _check(_lib.sw_revfft(C.c_size_t(m), C.c_size_t(n), _d(x), None if trig is None else _d(trig), fp))
+    _check(_lib.sw_revfft(C.c_size_t(m), C.c_size_t(n), _d(x), _d(trig), fp))
 
 
 # ---------------------------------------------------------------- sta2dfft (src/sta2dfft.rs)

@@ -275,12 +275,30 @@ int sw_initfft(size_t n, size_t factors[5], double* trig) {
     });
 }
 
-static int fft1d(size_t m, size_t n, double* x, const size_t factors[5], bool inv) {
+// forfft/revfft take the caller's trig table (stafft/mod.rs:158-162 asserts only its length); the kernels
+// use twiddles of their own, exp(-2 pi i k / n), so the caller's table must be initfft(n)'s, which is
+// checked value for value (host or device memory) to 1e-12: tables from the reference's initfft, whose
+// libm sin/cos may round differently, pass; a table of another length or content is an error.
+static void check_trig(size_t n, const double* trig) {
+    SW_CHECK(trig != nullptr, SW_ERR_ARG, "null trig");
+    size_t f[5];
+    std::vector<double> want(2 * n), got(2 * n);
+    SW_CHECK(factorise_ref(n, f), SW_ERR_FACTOR, "factorisation failed");
+    initfft_ref(n, f, want.data());
+    if (is_device_ptr(trig)) copy_d2h(got.data(), trig, 2 * n * sizeof(double), stream());
+    else std::memcpy(got.data(), trig, 2 * n * sizeof(double));
+    bool same = true;
+    for (size_t i = 0; i < 2 * n; i++) same = same && std::fabs(want[i] - got[i]) <= 1e-12;
+    SW_CHECK(same, SW_ERR_ARG, "trig is not initfft(n)'s table (the transforms of length n use that table)");
+}
+
+static int fft1d(size_t m, size_t n, double* x, const double* trig, const size_t factors[5], bool inv) {
     return guard([&] {
         SW_CHECK(x && m > 0 && n >= 2 && m <= (1u << 30) && n <= (1u << 16), SW_ERR_ARG, "bad forfft/revfft shape");
         size_t f[5];
         SW_CHECK(factorise_ref(n, f), SW_ERR_FACTOR, "factorisation failed");
         if (factors) SW_CHECK(std::memcmp(f, factors, sizeof(f)) == 0, SW_ERR_ARG, "factors do not match n");
+        check_trig(n, trig);
         SW_CHECK(reference_transform_is_dft(f), SW_ERR_UNSUPPORTED, "length not supported");
         hipStream_t st = stream();
         FftPlan<double> pl{};
@@ -405,12 +423,10 @@ int sw_fft_time_tile(size_t n, int bytes, int B, int nt, int nslab, int reps, do
 }
 
 int sw_forfft(size_t m, size_t n, double* x, const double* trig, const size_t factors[5]) {
-    (void)trig;
-    return fft1d(m, n, x, factors, false);
+    return fft1d(m, n, x, trig, factors, false);
 }
 int sw_revfft(size_t m, size_t n, double* x, const double* trig, const size_t factors[5]) {
-    (void)trig;
-    return fft1d(m, n, x, factors, true);
+    return fft1d(m, n, x, trig, factors, true);
 }
 
 int sw_d2fft_create(size_t nx, size_t ny, double lx, double ly, double* kx, double* ky, sw_d2fft** out) {

@@ -48,6 +48,42 @@ def test_forfft_closed_form_and_roundtrip(sw, n, m):
     close(xs.reshape(n, m), x, 4e-15 * np.log2(n))
 
 
+def test_trig_must_be_initffts_table(sw):
+    """forfft/revfft use the twiddles of initfft(n) (stafft/mod.rs:67-95): a table of another length or
+    content is refused, from host or device memory (device arrays allocated through the HIP runtime the
+    library is linked against), and the array is left untouched."""
+    import ctypes as C
+    n, m = 16, 4
+    _, trig = sw.initfft(n)
+    x = np.arange(n * m, dtype=np.float64)
+    sw.forfft(m, n, x.copy(), trig)
+    for bad in (sw.initfft(32)[1][:2 * n].copy(), trig * (1 + 1e-9)):
+        y = x.copy()
+        with pytest.raises(sw.SwError) as e:
+            sw.forfft(m, n, y, bad)
+        assert e.value.code == -1 and "initfft" in str(e.value)
+        assert np.array_equal(y, x)
+    hip = C.CDLL("libamdhip64.so.7")
+    hip.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+    hip.hipFree.argtypes = [C.c_void_p]
+    hip.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+    fwd = sw._lib.sw_forfft
+    fwd.argtypes = [C.c_size_t, C.c_size_t, C.c_void_p, C.c_void_p, C.c_void_p]
+    ptrs = []
+    try:
+        for a in (x, trig, trig + np.where(np.arange(2 * n) == 3, 1.0, 0.0)):
+            p = C.c_void_p()
+            assert hip.hipMalloc(C.byref(p), C.c_size_t(a.nbytes)) == 0
+            ptrs.append(p)
+            assert hip.hipMemcpy(p, a.ctypes.data_as(C.c_void_p), C.c_size_t(a.nbytes), 1) == 0
+        xd, td, bad = ptrs
+        assert fwd(m, n, xd, td, None) == 0
+        assert fwd(m, n, xd, bad, None) == -1
+    finally:
+        for p in ptrs:
+            hip.hipFree(p)
+
+
 def test_unsupported_lengths(sw):
     x = np.zeros(10 * 10)
     with pytest.raises(sw.SwError) as e:
