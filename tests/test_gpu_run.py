@@ -69,3 +69,34 @@ def test_run_strip_32x4_20_steps(sw, oracle):
     for f in ALL:
         assert rel_l2(g.get(f), getattr(o, f)) < 20 * STEP_TOL[f], f
 
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_run_128x16_wave_path(sw, oracle, precision):
+    """The time loop on the wave-kernel path with the fused psolve iteration (ng = 128, nz = 16): 12 steps
+    from a seeded k^-3 state (rms zeta = 0.3 f) with saves every 4, against the oracle.  FP64: equal psolve
+    iteration counts, monitor and energy rows at rtol 1e-10, fields within the step bounds times 12.  The
+    mixed mode (FP32 psolve iteration, its own iteration counts): monitor and energy rows within ten
+    times their measured difference from the FP64 oracle."""
+    ng, nz = 128, 16
+    qs, ds = sw.synth_kminus3(ng, nz, seed=7, k0=8.0, zeta_rms=0.3 * 4 * np.pi)
+    zero = np.zeros_like(qs)
+    g = sw.State(sw.Spectral(ng, nz, sw.FP64 if precision == "fp64" else sw.MIXED), qs=qs, ds=ds, gs=zero)
+    o = oracle.State(oracle.Spectral(ng, nz), qs=qs, ds=ds, gs=zero)
+    dt = 1.0 / ng
+    mon, eco, its = sw.run(g, 11.5 * dt, 4 * dt)
+    rc, mon2, eco2, its2 = oracle.run(o, 11.5 * dt, 4 * dt)
+    assert rc == 0 and len(mon) == len(mon2) == 12 and len(eco) == len(eco2) == 4
+    rel = lambda a, b: float(np.max(np.abs(a - b) / np.maximum(np.abs(b), 1e-300)))
+    print(f"{precision} 128x16, 12 steps: psolve iterations {its} (oracle {its2}), monitor rel {rel(mon, mon2):.1e}, "
+          f"ecomp rel {rel(eco, eco2):.1e}", flush=True)
+    if precision == "fp64":
+        assert its == its2
+        np.testing.assert_allclose(mon, mon2, rtol=1e-10, atol=1e-300)
+        np.testing.assert_allclose(eco, eco2, rtol=1e-10, atol=1e-300)
+        for f in ALL:
+            assert rel_l2(g.get(f), getattr(o, f)) < 12 * STEP_TOL[f], f
+    else:
+        # measured on MI355X: monitor 1.9e-8, ecomp 4.9e-9 (bounds ten times that, within the 1e-5 bar)
+        np.testing.assert_allclose(mon, mon2, rtol=2e-7, atol=1e-300)
+        np.testing.assert_allclose(eco, eco2, rtol=5e-8, atol=1e-300)
