@@ -1199,6 +1199,10 @@ struct Engine {
     // downstream of the inversion, vertical, cpsource and source amplify it by up to (ng/3)^2, whereas
     // the pressure iteration solves an elliptic problem, which damps it.
     Engine<float>* sub = nullptr;
+    // within psolve, after the FP32 iteration (nz with a compile-time specialisation): its ps and S,
+    // read by the launches that follow instead of FP64 copies (null otherwise)
+    const float* ps_f32 = nullptr;
+    const float* sptop_f32 = nullptr;
     void to_f32(const T* a, float* b, size_t n) {
         each(st, n, [=] __device__(size_t i) { b[i] = (float)a[i]; });
     }
