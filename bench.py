@@ -314,6 +314,32 @@ def ensemble_bench(sw, steps, warmup, nmember=64, runs=((0, "batched"), (1, "bat
     return out
 
 
+def hbm_ceiling():
+    """What plain torch read (sum), write (fill) and copy kernels reach on 1 GiB FP64 buffers, beyond the
+    256 MB last-level cache (HIP events, 20 launches each): the practical ceilings of HBM traffic with
+    those read/write mixes, beside the 8 TB/s peak that roofline.frac is quoted against."""
+    import torch
+    n = (1 << 30) // 8
+    a = torch.ones(n, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    out = {}
+    for name, fn, moved in (("read", lambda: a.sum(), n * 8), ("write", lambda: b.fill_(2.0), n * 8),
+                            ("copy", lambda: b.copy_(a), 2 * n * 8)):
+        for _ in range(2):
+            fn()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(20):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        t = e0.elapsed_time(e1) / 20 * 1e-3
+        out[name] = {"TB_per_s": moved / t / 1e12, "frac_of_peak": moved / t / (HBM_PEAK_GBS * 1e9)}
+    del a, b
+    torch.cuda.empty_cache()
+    return out
+
+
 def cpu_model():
     try:
         for line in open("/proc/cpuinfo"):
@@ -382,6 +408,9 @@ def main():
     ap.add_argument("--large", type=int, default=0, metavar="STEPS",
                     help="only the large mixed-radix workload (ng=1536, nz=24, k^-3 PV + random delta): STEPS steps "
                          "mixed and FP64, errors at step 1 and at the end")
+    ap.add_argument("--hbm-ceiling", action="store_true",
+                    help="only measure what plain read, write and copy kernels reach on 1 GiB buffers (beyond the "
+                         "256 MB last-level cache): the practical HBM ceilings beside the 8 TB/s peak")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (twice) and exit: the reference reading "
                          "for the FETCH_SIZE/WRITE_SIZE counters")
@@ -396,6 +425,9 @@ def main():
         torch.cuda.synchronize()
         print(f"[bench] copy calibration: 2 copies of {args.copy_calibration} MiB (read {args.copy_calibration} MiB "
               f"+ write {args.copy_calibration} MiB each)", file=sys.stderr, flush=True)
+        return
+    if args.hbm_ceiling:
+        print(json.dumps({"hbm_ceiling_1GiB": hbm_ceiling()}), flush=True)
         return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: the library has no multi-GPU path (ensemble members, or the "
@@ -464,6 +496,7 @@ def main():
     others = {k: {"kernel": names[k], "ms_per_step": v["ms_per_step"], "avg_launch_us": v["ms"] * 1e3 / max(v["launches"], 1),
                   "achieved_GBs": v["bytes"] / (v["ms"] * 1e-3) / 1e9 if v["ms"] > 0 else 0.0}
               for k, v in h["prof"].items()}
+    ceiling = hbm_ceiling()
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(sw, ng, nz, init)
@@ -502,7 +535,8 @@ def main():
                      "kernel": kname,
                      "launches": pk["launches"], "avg_launch_us": pk["ms"] * 1e3 / max(pk["launches"], 1),
                      "algorithmic_bytes_per_launch": pk["bytes"] / max(pk["launches"], 1),
-                     "profiled_kernels": others},
+                     "profiled_kernels": others,
+                     "hbm_ceiling_1GiB": ceiling},
         "ensemble": ens,
         "cpu_baseline": cpu,
     }
