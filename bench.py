@@ -340,6 +340,35 @@ def hbm_ceiling():
     return out
 
 
+def transform_rate(sw, ng=512, nslab=256, reps=20):
+    """The stand-alone FP64 2-D transforms (ptospc3d, spctop3d) of nslab slabs of ng^2 held in device memory
+    (537 MB in and out at the defaults, beyond the last-level cache): TB/s over both passes, each pass
+    reading and writing every value."""
+    import ctypes as C
+    import torch
+    n = ng * ng * nslab
+    spec = sw.Spectral(ng, nslab - 1)
+    a = torch.randn(n, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    out = {}
+    for name in ("ptospc3d", "spctop3d"):
+        fn = getattr(sw._lib, "sw_" + name)
+        fn.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t, C.c_size_t]
+        for _ in range(2):
+            assert fn(spec._h, a.data_ptr(), b.data_ptr(), 0, nslab - 1) == 0
+        sw.sync()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            fn(spec._h, a.data_ptr(), b.data_ptr(), 0, nslab - 1)
+        sw.sync()
+        t = (time.perf_counter() - t0) / reps
+        out[name] = {"us": t * 1e6, "TB_per_s": 4 * n * 8 / t / 1e12}
+    del spec, a, b
+    torch.cuda.empty_cache()
+    return out
+
+
 def cpu_model():
     try:
         for line in open("/proc/cpuinfo"):
@@ -427,13 +456,16 @@ def main():
               f"+ write {args.copy_calibration} MiB each)", file=sys.stderr, flush=True)
         return
     if args.hbm_ceiling:
-        print(json.dumps({"hbm_ceiling_1GiB": hbm_ceiling()}), flush=True)
+        print(json.dumps({"hbm_ceiling_1GiB": hbm_ceiling(), "transform_pair_537MB": transform_rate(load_sw())}),
+              flush=True)
         return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: the library has no multi-GPU path (ensemble members, or the "
               "levels of one member with an all-to-all per psolve iteration, would shard across GPUs); "
               "run with --gpus 1", file=sys.stderr)
         sys.exit(2)
+    # the HBM ceilings first: torch initialises its own HIP context before the library does
+    ceiling = hbm_ceiling() if not (args.large or args.ensemble_only or args.only) else None
     sw = load_sw()
     if args.large:
         # the bench state (rms zeta = f/2), on which the reference algorithm's pressure iteration stops
@@ -496,7 +528,6 @@ def main():
     others = {k: {"kernel": names[k], "ms_per_step": v["ms_per_step"], "avg_launch_us": v["ms"] * 1e3 / max(v["launches"], 1),
                   "achieved_GBs": v["bytes"] / (v["ms"] * 1e-3) / 1e9 if v["ms"] > 0 else 0.0}
               for k, v in h["prof"].items()}
-    ceiling = hbm_ceiling()
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(sw, ng, nz, init)
