@@ -1005,9 +1005,12 @@ __global__ void __launch_bounds__(192 * NG) fftw3_inv_pass(const C2<T>* __restri
 // comb.count(s) Hermitian inputs pro(s, k, r, f) (input k the derivative along the transform axis of input
 // k - 1 when comb.deriv_of_prev(s, k)) are transformed inverse, each folded at once into a = a +
 // comb.coef(s, k, r, j) val from a = comb.init(s, r, j), and the accumulated rows transformed forward and
-// stored transposed: element (r, f) at tout[r + N3 f + N3 N3 s].
+// stored transposed: element (r, f) at tout[r + N3 f + N3 N3 s].  In FP32 the registers are held to 80
+// (a few spill), so that two 12-wave blocks share a CU: 888 -> 752 us per launch at 1536x24, against one
+// block per CU at the 94 the compiler picks otherwise.
 template <typename T, int N, int NG, class Pro, class Comb>
-__global__ void __launch_bounds__(192 * NG) fftw3_inv_comb_fwd(const C2<T>* __restrict__ tw3, int nslab, Pro pro, Comb comb,
+__global__ void __launch_bounds__(192 * NG) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 6 : 1)))
+fftw3_inv_comb_fwd(const C2<T>* __restrict__ tw3, int nslab, Pro pro, Comb comb,
                                                               T* tout, Gate gate) {
     constexpr int E = WaveShape<N>::E, N3 = 3 * N, BPS = N3 / (2 * NG), HT = E / 2;
     extern __shared__ __align__(16) unsigned char lds_raw[];
