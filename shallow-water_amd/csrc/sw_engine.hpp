@@ -43,10 +43,10 @@ struct PtrTab {
 // Slabs of fftw_deriv_lines in K groups of `per` levels for each of M members: output slab
 // s = (g M + m) per + l is the derivative of level l of member m of src[g] along x (bit g of xmask set)
 // or along y, written to level l of member m of dst[g]; member m's levels start at m mstride.
-template <typename T, int K>
+template <typename T, int K, typename O = T>
 struct DerivIo {
     PtrTab<const T, K> src;
-    PtrTab<T, K> dst;
+    PtrTab<O, K> dst;  // outputs stored as O (rounded from T)
     int per;
     unsigned xmask;
     size_t nn;
@@ -63,7 +63,7 @@ struct DerivIo {
         const size_t o = off(s, g);
         return src[g] + o;
     }
-    __device__ __forceinline__ T* out(int s) const {
+    __device__ __forceinline__ O* out(int s) const {
         int g;
         const size_t o = off(s, g);
         return dst[g] + o;
@@ -1084,14 +1084,14 @@ struct Engine {
     // Horizontal derivatives of physical slabs (DerivIo) as 1-D transform pairs along their axis
     // (fftw_deriv_lines, one launch for all nslab outputs).  Returns false, launching nothing, when
     // ng has no wave kernels: the caller then forms them with a 2-D transform pair.
-    template <int K>
-    bool deriv_lines(int nslab, const DerivIo<T, K>& io) {
+    template <int K, typename O>
+    bool deriv_lines(int nslab, const DerivIo<T, K, O>& io) {
         if (wave3_n()) {
             if (nslab <= 0) return true;
             constexpr int N = 512, N3 = 1536;
             const size_t lds = wave3_lds_bytes<N, NG3, T>();
-            lds_optin((const void*)fftw3_deriv_lines<T, N, NG3, DerivIo<T, K>>, lds);
-            hipLaunchKernelGGL((fftw3_deriv_lines<T, N, NG3, DerivIo<T, K>>), dim3(nslab * (N3 / (2 * NG3))), dim3(192 * NG3), lds,
+            lds_optin((const void*)fftw3_deriv_lines<T, N, NG3, DerivIo<T, K, O>>, lds);
+            hipLaunchKernelGGL((fftw3_deriv_lines<T, N, NG3, DerivIo<T, K, O>>), dim3(nslab * (N3 / (2 * NG3))), dim3(192 * NG3), lds,
                                st, plan.tw, nslab, io);
             SW_HIP(hipGetLastError());
             return true;
@@ -1101,7 +1101,7 @@ struct Engine {
         wave_dispatch([&](auto nn_) {
             constexpr int NN = decltype(nn_)::value;
             const size_t lds = wave_lds_bytes<NN, NWT, T>();
-            hipLaunchKernelGGL((fftw_deriv_lines<T, NN, NWT, DerivIo<T, K>>), dim3(nslab * (NN / (2 * NWT))),
+            hipLaunchKernelGGL((fftw_deriv_lines<T, NN, NWT, DerivIo<T, K, O>>), dim3(nslab * (NN / (2 * NWT))),
                                dim3(64 * NWT), lds, st, plan.tw, nslab, io);
             SW_HIP(hipGetLastError());
         });
@@ -1203,6 +1203,9 @@ struct Engine {
     // read by the launches that follow instead of FP64 copies (null otherwise)
     const float* ps_f32 = nullptr;
     const float* sptop_f32 = nullptr;
+    // J(u, v) = ux vy - uy vx on every level, formed by cpsource when it stores the derivatives of u, v in
+    // FP32 (the mixed mode), for source's reuse of them (null otherwise)
+    const T* juv_src = nullptr;
     void to_f32(const T* a, float* b, size_t n) {
         each(st, n, [=] __device__(size_t i) { b[i] = (float)a[i]; });
     }

@@ -565,7 +565,7 @@ __global__ void __launch_bounds__(64 * NW) fftw_deriv_lines(const C2<T>* __restr
     if (s >= nslab) return;
     const bool ax = io.along_x(s);
     const T* __restrict__ in = io.in(s);
-    T* __restrict__ out = io.out(s);
+    auto* __restrict__ out = io.out(s);  // T, or a narrower type the values are rounded to
 
     C2<T> v[E];
     if (ax) {
@@ -1154,7 +1154,7 @@ __global__ void __launch_bounds__(192 * NG) fftw3_deriv_lines(const C2<T>* __res
     if (s >= nslab) return;
     const bool ax = io.along_x(s);
     const T* __restrict__ in = io.in(s);
-    T* __restrict__ out = io.out(s);
+    auto* __restrict__ out = io.out(s);  // T, or a narrower type the values are rounded to
 
     C2<T> v[E];
     if (ax) {
