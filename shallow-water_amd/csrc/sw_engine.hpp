@@ -964,12 +964,12 @@ struct Engine {
                (prof == SW_PROF_PSOLVE_TRI && site == SITE_PSOLVE_TRI);
     }
     // CUT: output rows whose wavenumber exceeds kcut are not stored
-    template <int NN, int NW, bool CUT = false, class Pro>
-    void fwd_pass1_wave(int nslab, Pro pro, T* out, int site, int kcut = NN / 2) {
+    template <int NN, int NW, bool CUT = false, class Pro, typename TO>
+    void fwd_pass1_wave(int nslab, Pro pro, TO* out, int site, int kcut = NN / 2) {
         const size_t lds_t = wave_lds_bytes<NN, NW, T>();
         const bool tm = timed(site);
         if (tm) prof_start(2.0 * sizeof(T) * (double)n2 * nslab);
-        hipLaunchKernelGGL((fftw_fwd_pass<T, NN, NW, true, Pro, NoEpi, CUT>), dim3(nslab * (NN / (2 * NW))), dim3(64 * NW),
+        hipLaunchKernelGGL((fftw_fwd_pass<T, NN, NW, true, Pro, NoEpi, CUT, TO>), dim3(nslab * (NN / (2 * NW))), dim3(64 * NW),
                            lds_t, st, plan.tw, nslab, pro, NoEpi{}, out, kcut);
         if (tm) prof_stop();
         SW_HIP(hipGetLastError());

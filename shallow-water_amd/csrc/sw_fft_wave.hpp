@@ -202,10 +202,10 @@ __device__ __forceinline__ void wave_split_pair(const C2<T> (&v)[WaveShape<N>::E
 // stage [f][row] so that each output row gets a run of 2 NW consecutive values.  o = slab base + r0.
 // CUT: output rows f whose wavenumber min(f, N - f) exceeds kcut are not stored (a band that the next
 // pass treats as zero).
-template <int N, int NW, bool CUT = false, typename T>
+template <int N, int NW, bool CUT = false, typename T, typename O>
 __device__ __forceinline__ void wave_store_trans(const T (&ar)[WaveShape<N>::E / 2 + 1], const T (&ai)[WaveShape<N>::E / 2 + 1],
                                                  const T (&br)[WaveShape<N>::E / 2 + 1], const T (&bi)[WaveShape<N>::E / 2 + 1],
-                                                 T* stg, int lane, int wv, T* o, int kcut = N / 2) {
+                                                 T* stg, int lane, int wv, O* o, int kcut = N / 2) {
     constexpr int E = WaveShape<N>::E, NT = E / 2 + 1, RB = 2 * NW, RS = stage_stride<NW>();
     static_assert(NW > 1, "a transposed store stages 2 NW rows through the block's LDS");
     __syncthreads();
@@ -246,9 +246,9 @@ __device__ __forceinline__ void wave_store_trans(const T (&ar)[WaveShape<N>::E /
 // Forward (real -> Hermitian) pass; pro(s, r, j) gives row r, element j.  TRANS: element (r, f) goes
 // to out[r + n f + n n s] (with CUT, rows f with min(f, N - f) > kcut left unwritten: see
 // wave_store_trans); otherwise epi(s, r, f, value).
-template <typename T, int N, int NW, bool TRANS, class Pro, class Epi, bool CUT = false>
+template <typename T, int N, int NW, bool TRANS, class Pro, class Epi, bool CUT = false, typename TO = T>
 __global__ void __launch_bounds__(64 * NW) fftw_fwd_pass(const C2<T>* __restrict__ tw, int nslab, Pro pro, Epi epi,
-                                                        T* tout, int kcut) {
+                                                        TO* tout, int kcut) {
     constexpr int E = WaveShape<N>::E, BPS = N / (2 * NW), NT = E / 2 + 1;
     extern __shared__ __align__(16) unsigned char lds_raw[];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
@@ -289,9 +289,9 @@ __global__ void __launch_bounds__(64 * NW) fftw_fwd_pass(const C2<T>* __restrict
 // A + sig(s) D(C), D the derivative along the transform axis (d[f] = dmul(f) x[mir(f)]: cosine part
 // -k C'_k, sine part k C_k, zero at k = 0 and N/2).  A wave owns rows ra and ra + 1, one transform
 // each.  TRANS: element (r, f) goes to out[r + n f + n n s]; otherwise epi(s, r, f, value).
-template <typename T, int N, int NW, bool TRANS, class Pro, class Sig, class Epi>
+template <typename T, int N, int NW, bool TRANS, class Pro, class Sig, class Epi, typename TO = T>
 __global__ void __launch_bounds__(64 * NW) fftw_fwd_dcomb(const C2<T>* __restrict__ tw, int nslab, Pro pro, Sig sig,
-                                                         Epi epi, T* tout) {
+                                                         Epi epi, TO* tout) {
     constexpr int E = WaveShape<N>::E, BPS = N / (2 * NW), NT = E / 2 + 1;
     extern __shared__ __align__(16) unsigned char lds_raw[];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
