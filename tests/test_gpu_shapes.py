@@ -48,7 +48,8 @@ def report(tag, errs, iters=None):
     worst = max(errs, key=lambda f: errs[f] / STEP_TOL[f])
     print(f"{tag}: psolve iterations {iters}; rel L2 vs oracle "
           + " ".join(f"{f}={e:.1e}" for f, e in errs.items())
-          + f"; largest against its bound: {worst} ({errs[worst] / STEP_TOL[worst]:.2f} of it)", flush=True)
+          + f"; largest against the small-shape step bounds of _tolerances.py (asserted there only for 128): "
+          + f"{worst} ({errs[worst] / STEP_TOL[worst]:.2f} of it)", flush=True)
 
 
 def one_step(sw, oracle, ng, nz, sensitivity=True, **kw):
@@ -79,7 +80,7 @@ def one_step(sw, oracle, ng, nz, sensitivity=True, **kw):
 
 def assert_within_conditioning(errs, sens):
     # the bar per field: 1e-12, or ten times the reference algorithm's own sensitivity to 1e-16 noise in
-    # its input where that is larger (measured GPU error / sensitivity: 1.0-1.6 at 512x16)
+    # its input where that is larger (measured GPU error / sensitivity: 0.96-2.6 at 512x16)
     for f in ALL:
         assert errs[f] < max(1e-12, 10 * sens[f]), (f, errs[f], sens[f])
 
